@@ -1,0 +1,523 @@
+"""Synthetic JPL-style ephemeris and particle initial conditions.
+
+The real DE440 / sb441-n16 kernels are not available, so every test and
+benchmark runs on a deterministic synthetic ephemeris with the same shape:
+
+* Sun, Mercury..Neptune and Pluto barycentres, the Earth-Moon barycentre (EMB),
+  Earth and Moon relative to the EMB, plus 16 asteroids relative to the Sun.
+* Planets move on heliocentric Kepler orbits with planet-like elements; the Sun
+  moves so that the solar-system barycentre stays at the origin.  The Moon moves
+  on a geocentric Kepler orbit; Earth and Moon are placed about the EMB with the
+  Earth/Moon mass ratio EMRAT.
+* Positions are fitted to Chebyshev records (planets 32 d / 14 coefficients,
+  Moon and Earth-wrt-EMB 4 d / 13, asteroids 16 d / 11) and written as DAF/SPK
+  type-2 files in the layout the reference reader expects
+  (src/spk.c:267-402 summaries and mmap, :405-547 record evaluation,
+  :696-766 comment-area constants).  Positions are in km and epochs in TDB
+  seconds past J2000, as in real SPK files.
+
+Everything is numpy float64 and seeded, so the files are bit-identical on every
+machine with the same numpy build.
+"""
+from __future__ import annotations
+
+import math
+import os
+import struct
+from dataclasses import dataclass
+
+import numpy as np
+
+AU_KM = 149597870.7
+DAY_S = 86400.0
+JD_J2000 = 2451545.0
+OBLIQUITY_DEG = 23.439281
+
+# Constants written into the comment area (real-scale magnitudes; GM in au^3/d^2).
+CONSTANTS = {
+    "AU": AU_KM,
+    "EMRAT": 81.3005682214972154,
+    "J2E": 1.0826253900e-3,
+    "J3E": -2.5324100000e-6,
+    "J4E": -1.6198980000e-6,
+    "J2SUN": 2.1961391516529825e-7,
+    "RE": 6378.1363,
+    "CLIGHT": 299792.458,
+    "ASUN": 696000.0,
+}
+GM_PLANETS = {
+    "GMS": 2.9591220828411956e-04,
+    "GM1": 4.9125001948893182e-11,
+    "GM2": 7.2434523326441187e-10,
+    "GMB": 8.9970114082680434e-10,
+    "GM4": 9.5495488297258119e-11,
+    "GM5": 2.8253458252257917e-07,
+    "GM6": 8.4597059933762903e-08,
+    "GM7": 1.2920265649682399e-08,
+    "GM8": 1.5243573478851939e-08,
+    "GM9": 2.1750964648933581e-12,
+}
+
+# Heliocentric ecliptic J2000 elements: a [au], e, i, node, long. perihelion,
+# mean longitude at J2000 [deg] (planet-like values).
+PLANET_ELEMENTS = {
+    # naif: (name, GM key, a, e, i, node, varpi, L)
+    1: ("Mercury", "GM1", 0.38709927, 0.20563593, 7.00497902, 48.33076593, 77.45779628, 252.25032350),
+    2: ("Venus", "GM2", 0.72333566, 0.00677672, 3.39467605, 76.67984255, 131.60246718, 181.97909950),
+    3: ("EMB", "GMB", 1.00000261, 0.01671123, 0.00001531, 11.26064, 102.93768193, 100.46457166),
+    4: ("Mars", "GM4", 1.52371034, 0.09339410, 1.84969142, 49.55953891, 336.05637041, 355.44656795),
+    5: ("Jupiter", "GM5", 5.20288700, 0.04838624, 1.30439695, 100.47390909, 14.72847983, 34.39644051),
+    6: ("Saturn", "GM6", 9.53667594, 0.05386179, 2.48599187, 113.66242448, 92.59887831, 49.95424423),
+    7: ("Uranus", "GM7", 19.18916464, 0.04725744, 0.77263783, 74.01692503, 170.95427630, 313.23810451),
+    8: ("Neptune", "GM8", 30.06992276, 0.00859048, 1.77004347, 131.78422574, 44.96476227, 304.87997031),
+    9: ("Pluto", "GM9", 39.48211675, 0.24882730, 17.14001206, 110.30393684, 224.06891629, 238.92903833),
+}
+# Geocentric ecliptic elements of the Moon: a [km], e, i, node, argperi, M0 [deg].
+MOON_ELEMENTS = (384400.0, 0.0549, 5.145, 125.08, 318.15, 135.27)
+
+# The 16 perturbing asteroids in ASSIST body order 11..26 (alphabetical by name,
+# assist/ephem.py:20-35): (name, number, GM [au^3/d^2], a, e, i).
+ASTEROIDS = [
+    ("Camilla", 107, 1.6e-15, 3.48, 0.065, 10.0),
+    ("Ceres", 1, 1.3964518123081070e-13, 2.767, 0.0785, 10.59),
+    ("Cybele", 65, 2.2e-15, 3.43, 0.11, 3.56),
+    ("Davida", 511, 2.6e-15, 3.17, 0.19, 15.94),
+    ("Eunomia", 15, 4.5e-15, 2.64, 0.187, 11.75),
+    ("Euphrosyne", 31, 2.5e-15, 3.15, 0.22, 26.3),
+    ("Europa", 52, 3.5e-15, 3.10, 0.11, 7.48),
+    ("Hygiea", 10, 1.2542530761640810e-14, 3.14, 0.112, 3.83),
+    ("Interamnia", 704, 5.2e-15, 3.06, 0.155, 17.3),
+    ("Iris", 7, 2.0e-15, 2.39, 0.23, 5.52),
+    ("Juno", 3, 4.3e-15, 2.67, 0.256, 12.99),
+    ("Pallas", 2, 3.0471146330043200e-14, 2.77, 0.23, 34.8),
+    ("Psyche", 16, 3.5e-15, 2.92, 0.134, 3.1),
+    ("Sylvia", 87, 2.2e-15, 3.49, 0.09, 10.9),
+    ("Thisbe", 88, 1.4e-15, 2.77, 0.16, 5.2),
+    ("Vesta", 4, 3.8548000225257904e-14, 2.36, 0.089, 7.14),
+]
+
+# Record layout per SPK target: (record length [days], coefficients per coordinate)
+PLANET_RECORD = (32.0, 14)
+MOON_RECORD = (4.0, 13)
+ASTEROID_RECORD = (16.0, 11)
+COVERAGE_DAYS = 10000.0  # JD 2451545 +/- this
+
+
+# --------------------------------------------------------------------------------------
+# Kepler orbits
+# --------------------------------------------------------------------------------------
+
+def _solve_kepler(M, e):
+    """Eccentric anomaly E for mean anomaly M (radians, any shape), Newton iteration."""
+    M = np.asarray(M, dtype=np.float64)
+    e = np.asarray(e, dtype=np.float64)
+    E = M + e * np.sin(M) * (1.0 + e * np.cos(M))
+    for _ in range(60):
+        f = E - e * np.sin(E) - M
+        E_new = E - f / (1.0 - e * np.cos(E))
+        if np.all(np.abs(E_new - E) <= 1e-15 * np.maximum(1.0, np.abs(E_new))):
+            E = E_new
+            break
+        E = E_new
+    return E
+
+
+def _rot_ecl_to_eq():
+    eps = math.radians(OBLIQUITY_DEG)
+    c, s = math.cos(eps), math.sin(eps)
+    return np.array([[1.0, 0.0, 0.0], [0.0, c, -s], [0.0, s, c]])
+
+
+def kepler_state(mu, a, e, inc, node, argp, M, equatorial=True):
+    """Position and velocity (arrays of shape (...,3)) from classical elements.
+
+    Angles in radians; the frame is ecliptic J2000, rotated to equatorial J2000
+    (the SPK frame) when ``equatorial``.
+    """
+    a, e, inc, node, argp, M = np.broadcast_arrays(*(np.asarray(v, dtype=np.float64) for v in (a, e, inc, node, argp, M)))
+    E = _solve_kepler(M, e)
+    cE, sE = np.cos(E), np.sin(E)
+    b = a * np.sqrt(1.0 - e * e)
+    xp = a * (cE - e)
+    yp = b * sE
+    n = np.sqrt(mu / a**3)
+    Edot = n / (1.0 - e * cE)
+    vxp = -a * sE * Edot
+    vyp = b * cE * Edot
+    cO, sO = np.cos(node), np.sin(node)
+    cw, sw = np.cos(argp), np.sin(argp)
+    ci, si = np.cos(inc), np.sin(inc)
+    P = np.stack([cO * cw - sO * sw * ci, sO * cw + cO * sw * ci, sw * si], axis=-1)
+    Q = np.stack([-cO * sw - sO * cw * ci, -sO * sw + cO * cw * ci, cw * si], axis=-1)
+    r = xp[..., None] * P + yp[..., None] * Q
+    v = vxp[..., None] * P + vyp[..., None] * Q
+    if equatorial:
+        R = _rot_ecl_to_eq()
+        r = r @ R.T
+        v = v @ R.T
+    return r, v
+
+
+def state_to_elements(mu, r, v, equatorial=True):
+    """Inverse of :func:`kepler_state` for elliptic orbits: (a, e, inc, node, argp, M)."""
+    r = np.asarray(r, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    if equatorial:
+        R = _rot_ecl_to_eq()
+        r = r @ R
+        v = v @ R
+    rn = np.linalg.norm(r, axis=-1)
+    h = np.cross(r, v)
+    hn = np.linalg.norm(h, axis=-1)
+    inc = np.arccos(np.clip(h[..., 2] / hn, -1, 1))
+    node = np.arctan2(h[..., 0], -h[..., 1])
+    evec = np.cross(v, h) / mu - r / rn[..., None]
+    e = np.linalg.norm(evec, axis=-1)
+    energy = 0.5 * np.sum(v * v, axis=-1) - mu / rn
+    a = -mu / (2 * energy)
+    nvec = np.stack([np.cos(node), np.sin(node), np.zeros_like(node)], axis=-1)
+    argp = np.arctan2(np.sum(np.cross(nvec, evec) * h, axis=-1) / hn, np.sum(nvec * evec, axis=-1))
+    nu = np.arctan2(np.sum(np.cross(evec, r) * h, axis=-1) / hn, np.sum(evec * r, axis=-1))
+    E = 2 * np.arctan2(np.sqrt(1 - e) * np.sin(nu / 2), np.sqrt(1 + e) * np.cos(nu / 2))
+    M = E - e * np.sin(E)
+    return a, e, inc, np.mod(node, 2 * np.pi), np.mod(argp, 2 * np.pi), np.mod(M, 2 * np.pi)
+
+
+@dataclass
+class SyntheticModel:
+    """Analytic body positions of the synthetic solar system (seeded)."""
+
+    seed: int = 0
+
+    def __post_init__(self):
+        rng = np.random.default_rng(self.seed)
+        self.gms = GM_PLANETS["GMS"]
+        self.emrat = CONSTANTS["EMRAT"]
+        self.planets = {}
+        for naif, (name, key, a, e, i, node, varpi, L) in PLANET_ELEMENTS.items():
+            gm = GM_PLANETS[key]
+            argp = math.radians(varpi - node)
+            M0 = math.radians(L - varpi)
+            self.planets[naif] = dict(gm=gm, a=a, e=e, i=math.radians(i), node=math.radians(node), argp=argp, M0=M0)
+        self.asteroids = []
+        for name, num, gm, a, e, i in ASTEROIDS:
+            node, argp, M0 = rng.uniform(0.0, 2 * np.pi, size=3)
+            self.asteroids.append(dict(name=name, num=num, gm=gm, a=a, e=e, i=math.radians(i), node=node, argp=argp, M0=M0))
+        a, e, i, node, argp, M0 = MOON_ELEMENTS
+        self.moon = dict(a=a / AU_KM, e=e, i=math.radians(i), node=math.radians(node), argp=math.radians(argp), M0=math.radians(M0))
+
+    # All positions in au, velocities au/d, times in days relative to J2000 (TDB).
+    def _helio_planet(self, naif, t):
+        p = self.planets[naif]
+        mu = self.gms + p["gm"]
+        n = math.sqrt(mu / p["a"] ** 3)
+        return kepler_state(mu, p["a"], p["e"], p["i"], p["node"], p["argp"], p["M0"] + n * np.asarray(t))
+
+    def sun(self, t):
+        """Barycentric Sun state: the barycentre of Sun + planets stays at the origin."""
+        t = np.asarray(t, dtype=np.float64)
+        tot = self.gms
+        rs = np.zeros(t.shape + (3,))
+        vs = np.zeros(t.shape + (3,))
+        for naif, p in self.planets.items():
+            r, v = self._helio_planet(naif, t)
+            rs -= p["gm"] * r
+            vs -= p["gm"] * v
+            tot += p["gm"]
+        return rs / tot, vs / tot
+
+    def planet(self, naif, t):
+        rs, vs = self.sun(t)
+        r, v = self._helio_planet(naif, t)
+        return rs + r, vs + v
+
+    def moon_geocentric(self, t):
+        m = self.moon
+        mu = GM_PLANETS["GMB"]
+        n = math.sqrt(mu / m["a"] ** 3)
+        return kepler_state(mu, m["a"], m["e"], m["i"], m["node"], m["argp"], m["M0"] + n * np.asarray(t))
+
+    def earth_wrt_emb(self, t):
+        r, v = self.moon_geocentric(t)
+        f = 1.0 / (1.0 + self.emrat)
+        return -f * r, -f * v
+
+    def moon_wrt_emb(self, t):
+        r, v = self.moon_geocentric(t)
+        f = self.emrat / (1.0 + self.emrat)
+        return f * r, f * v
+
+    def asteroid_helio(self, k, t):
+        p = self.asteroids[k]
+        mu = self.gms
+        n = math.sqrt(mu / p["a"] ** 3)
+        return kepler_state(mu, p["a"], p["e"], p["i"], p["node"], p["argp"], p["M0"] + n * np.asarray(t))
+
+    def target_state(self, code, t):
+        """SPK target state relative to its SPK centre, in au and au/d."""
+        if code == 10:
+            return self.sun(t)
+        if code in self.planets:
+            return self.planet(code, t)
+        if code == 399:
+            return self.earth_wrt_emb(t)
+        if code == 301:
+            return self.moon_wrt_emb(t)
+        if code >= 2000000:
+            for k, a in enumerate(self.asteroids):
+                if a["num"] == code - 2000000:
+                    return self.asteroid_helio(k, t)
+        raise KeyError(code)
+
+
+# --------------------------------------------------------------------------------------
+# Chebyshev fitting
+# --------------------------------------------------------------------------------------
+
+def chebyshev_fit_records(fn, t_start, rec_days, nrec, ncoef):
+    """Fit ``fn(t) -> positions [km]`` on consecutive records by interpolation at
+    Chebyshev-Gauss nodes.  Returns (mid_s, radius_s, coeffs[nrec, 3, ncoef])."""
+    k = np.arange(ncoef)
+    theta = np.pi * (k + 0.5) / ncoef
+    znodes = np.cos(theta)
+    mids = t_start + rec_days * (np.arange(nrec) + 0.5)
+    half = 0.5 * rec_days
+    tt = mids[:, None] + half * znodes[None, :]
+    pos = fn(tt)  # (nrec, ncoef, 3) in km
+    # c_j = (2/P) sum_k f(z_k) cos(j theta_k); c_0 halved
+    cosm = np.cos(np.outer(k, theta))  # (j, k)
+    coeffs = (2.0 / ncoef) * np.einsum("jk,rkc->rcj", cosm, pos)
+    coeffs[:, :, 0] *= 0.5
+    return mids * DAY_S, np.full(nrec, half * DAY_S), coeffs
+
+
+# --------------------------------------------------------------------------------------
+# DAF/SPK writer
+# --------------------------------------------------------------------------------------
+
+RECORD_BYTES = 1024
+WORDS_PER_RECORD = 128
+_FTPSTR = b"FTPSTR:\r:\n:\r\n:\r\x00:\x81:\x10\xce:ENDFTP"
+
+
+@dataclass
+class SpkSegment:
+    target: int
+    center: int
+    init_s: float
+    intlen_s: float
+    mids: np.ndarray
+    radii: np.ndarray
+    coeffs: np.ndarray  # (nrec, 3, P) km
+
+
+def _fmt_d(x):
+    """Fortran style 'D' exponent with 17 significant digits."""
+    return ("%.16E" % x).replace("E", "D")
+
+
+def _comment_records(lines):
+    """Pack text lines into 1024-byte comment records.
+
+    Lines end with NUL (the DAF convention).  The reference reader strips trailing
+    NUL/EOT bytes of every record (src/spk.c:87-91), so each record is padded with
+    spaces after its last complete line, and only the final record carries the EOT
+    marker followed by NUL padding."""
+    recs = []
+    cur = b""
+    for ln in lines:
+        b = ln.encode("ascii") + b"\x00"
+        if len(cur) + len(b) > 1000:
+            recs.append(cur + b" " * (RECORD_BYTES - len(cur)))
+            cur = b""
+        cur += b
+    cur += b"\x04"
+    recs.append(cur + b"\x00" * (RECORD_BYTES - len(cur)))
+    return recs
+
+
+def write_spk(path, segments, comment_lines, ifname="ASSIST-AMD SYNTHETIC"):
+    """Write a little-endian DAF/SPK file (ND=2, NI=6) with type-2 segments."""
+    comments = _comment_records(comment_lines)
+    n_comment = len(comments)
+    if len(segments) > 25:
+        raise ValueError("at most 25 segments per summary record are written")
+    fward = 2 + n_comment  # record number of the summary record
+    name_rec = fward + 1
+    first_data_word = (name_rec) * WORDS_PER_RECORD + 1
+    data = []
+    summaries = []
+    word = first_data_word
+    for seg in segments:
+        nrec, _, P = seg.coeffs.shape
+        rsize = 2 + 3 * P
+        recs = np.empty((nrec, rsize))
+        recs[:, 0] = seg.mids
+        recs[:, 1] = seg.radii
+        recs[:, 2:] = seg.coeffs.reshape(nrec, 3 * P)
+        arr = np.concatenate([recs.ravel(), [seg.init_s, seg.intlen_s, float(rsize), float(nrec)]])
+        beg_s = seg.init_s
+        end_s = seg.init_s + seg.intlen_s * nrec
+        summaries.append((beg_s, end_s, seg.target, seg.center, 1, 2, word, word + arr.size - 1))
+        data.append(arr)
+        word += arr.size
+    free_word = word
+    with open(path, "wb") as f:
+        fr = bytearray(RECORD_BYTES)
+        fr[0:8] = b"DAF/SPK "
+        struct.pack_into("<ii", fr, 8, 2, 6)
+        fr[16:76] = ifname.ljust(60).encode("ascii")[:60]
+        struct.pack_into("<iii", fr, 76, fward, fward, free_word)
+        fr[88:96] = b"LTL-IEEE"
+        fr[699:699 + len(_FTPSTR)] = _FTPSTR
+        f.write(fr)
+        for c in comments:
+            f.write(c)
+        sr = bytearray(RECORD_BYTES)
+        struct.pack_into("<ddd", sr, 0, 0.0, 0.0, float(len(summaries)))
+        off = 24
+        for s in summaries:
+            struct.pack_into("<ddiiiiii", sr, off, *s)
+            off += 40
+        f.write(sr)
+        nr = bytearray(b" " * RECORD_BYTES)
+        f.write(nr)
+        for arr in data:
+            f.write(arr.astype("<f8").tobytes())
+        pad = (-(free_word - 1) * 8) % RECORD_BYTES
+        f.write(b"\x00" * pad)
+
+
+def _segment(model, code, center, rec):
+    rec_days, ncoef = rec
+    t0 = -COVERAGE_DAYS
+    nrec = int(round(2 * COVERAGE_DAYS / rec_days))
+
+    def fn(tt):
+        r, _ = model.target_state(code, tt)
+        return r * AU_KM
+
+    mids, radii, coeffs = chebyshev_fit_records(fn, t0, rec_days, nrec, ncoef)
+    return SpkSegment(code, center, t0 * DAY_S, rec_days * DAY_S, mids, radii, coeffs)
+
+
+def comment_lines(model):
+    lines = [
+        "ASSIST-AMD synthetic ephemeris (seed %d): Kepler orbits fitted with Chebyshev records." % model.seed,
+        "Not a physical ephemeris; for tests and benchmarks only.",
+        "",
+        "Initial conditions and constants used for integration:",
+    ]
+    for k, v in CONSTANTS.items():
+        lines.append("%-8s %s" % (k, _fmt_d(v)))
+    for k, v in GM_PLANETS.items():
+        lines.append("%-8s %s" % (k, _fmt_d(v)))
+    for a in model.asteroids:
+        lines.append("MA%04d   %s" % (a["num"], _fmt_d(a["gm"])))
+    return lines
+
+
+def write_synthetic_ephemeris(planets_path, asteroids_path, seed=0):
+    """Write the synthetic planets and asteroid SPK files; returns the model."""
+    model = SyntheticModel(seed)
+    planet_segs = [_segment(model, 10, 0, PLANET_RECORD)]
+    for naif in (1, 2, 3, 4, 5, 6, 7, 8, 9):
+        planet_segs.append(_segment(model, naif, 0, PLANET_RECORD))
+    planet_segs.append(_segment(model, 399, 3, MOON_RECORD))
+    planet_segs.append(_segment(model, 301, 3, MOON_RECORD))
+    lines = comment_lines(model)
+    write_spk(planets_path, planet_segs, lines, "ASSIST-AMD SYNTHETIC PLANETS")
+    ast_segs = [_segment(model, 2000000 + a["num"], 10, ASTEROID_RECORD) for a in model.asteroids]
+    write_spk(asteroids_path, ast_segs, lines[:2], "ASSIST-AMD SYNTHETIC ASTEROIDS")
+    return model
+
+
+def default_paths():
+    root = os.environ.get("ASSIST_AMD_DATA", os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "data"))
+    return os.path.join(root, "synthetic_planets.bsp"), os.path.join(root, "synthetic_asteroids.bsp")
+
+
+def ensure_synthetic_ephemeris(seed=0):
+    """Paths of the seed-``seed`` synthetic files, generating them when absent."""
+    pp, ap = default_paths()
+    if seed != 0:
+        pp = pp.replace(".bsp", "_s%d.bsp" % seed)
+        ap = ap.replace(".bsp", "_s%d.bsp" % seed)
+    if not (os.path.exists(pp) and os.path.exists(ap)):
+        os.makedirs(os.path.dirname(pp), exist_ok=True)
+        tmp_p, tmp_a = pp + ".tmp%d" % os.getpid(), ap + ".tmp%d" % os.getpid()
+        write_synthetic_ephemeris(tmp_p, tmp_a, seed)
+        os.replace(tmp_p, pp)
+        os.replace(tmp_a, ap)
+    return pp, ap
+
+
+# --------------------------------------------------------------------------------------
+# Particle populations (heliocentric elements -> barycentric equatorial states)
+# --------------------------------------------------------------------------------------
+
+def _helio_to_bary(model, t0, r, v):
+    rs, vs = model.sun(np.asarray(t0, dtype=np.float64))
+    return r + rs, v + vs
+
+
+def particles_from_elements(model, t0, a, e, inc_deg, node, argp, M):
+    mu = model.gms
+    r, v = kepler_state(mu, a, e, np.radians(inc_deg), node, argp, M)
+    r, v = _helio_to_bary(model, t0, r, v)
+    return np.concatenate([r, v], axis=-1)
+
+
+def config1_particle(model, t0=0.0, seed=0):
+    """One test particle: a=2.7 au, e=0.08, i=10 deg, angles from the seed."""
+    rng = np.random.default_rng(seed)
+    node, argp, M = rng.uniform(0, 2 * np.pi, size=3)
+    return particles_from_elements(model, t0, 2.7, 0.08, 10.0, node, argp, M)
+
+
+def main_belt(model, n, rng, t0=0.0):
+    a = rng.uniform(1.8, 3.5, n)
+    e = rng.uniform(0.0, 0.3, n)
+    i = rng.uniform(0.0, 30.0, n)
+    node, argp, M = (rng.uniform(0, 2 * np.pi, n) for _ in range(3))
+    return particles_from_elements(model, t0, a, e, i, node, argp, M)
+
+
+def neo_like(model, n, rng, t0=0.0):
+    a = rng.uniform(0.8, 2.0, n)
+    e = rng.uniform(0.1, 0.6, n)
+    i = rng.uniform(0.0, 40.0, n)
+    node, argp, M = (rng.uniform(0, 2 * np.pi, n) for _ in range(3))
+    return particles_from_elements(model, t0, a, e, i, node, argp, M)
+
+
+def variational_identity(n, n_var=6):
+    """Initial variational vectors: the first ``n_var`` columns of the 6x6 identity."""
+    d = np.zeros((n, n_var, 6))
+    for k in range(n_var):
+        d[:, k, k] = 1.0
+    return d
+
+
+def mixed_population(model, n, rng, neo_fraction=0.3, t0=0.0):
+    n_neo = int(round(neo_fraction * n))
+    mb = main_belt(model, n - n_neo, rng, t0)
+    neo = neo_like(model, n_neo, rng, t0)
+    states = np.concatenate([mb, neo], axis=0)
+    perm = rng.permutation(n)
+    return states[perm], (perm >= n - n_neo)
+
+
+def marsden_params(n, rng, fraction=0.1):
+    """A1,A2,A3 log-uniform in [1e-15, 1e-11] au/d^2 for a ``fraction`` of particles."""
+    p = np.zeros((n, 3))
+    sel = rng.random(n) < fraction
+    k = int(sel.sum())
+    p[sel] = 10.0 ** rng.uniform(-15.0, -11.0, size=(k, 3))
+    return p
+
+
+def log_uniform_spans(n, rng, lo=30.0, hi=10000.0, backward_fraction=0.5):
+    span = np.exp(rng.uniform(math.log(lo), math.log(hi), n))
+    sign = np.where(rng.random(n) < backward_fraction, -1.0, 1.0)
+    return span * sign
