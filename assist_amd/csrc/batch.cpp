@@ -1,0 +1,565 @@
+// Batched GPU entry points of libassist, the device ephemeris, and the
+// simulation hooks (force evaluation and integration) built on them.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "device_types.h"
+#include "internal.h"
+#include "kernels.h"
+
+using namespace assist_gpu;
+
+namespace assist_gpu {
+
+struct DeviceEphemCache {
+    int device = -1;
+    double jd_ref = 0;
+    const spk_s* asteroids = nullptr;  // the asteroid file the tables were built with
+    DevEphem host;                     // host copy with device pointers
+    DevEphem* d_ephem = nullptr;
+    double* d_records = nullptr;
+};
+
+void free_device_ephem(DeviceEphemCache* c) {
+    if (!c) return;
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && c->device >= 0) {
+        (void)hipSetDevice(c->device);
+        if (c->d_ephem) (void)hipFree(c->d_ephem);
+        if (c->d_records) (void)hipFree(c->d_records);
+        (void)hipSetDevice(cur);
+    }
+    delete c;
+}
+
+}  // namespace assist_gpu
+
+namespace {
+
+std::mutex g_mutex;
+
+inline double jul(double eph) { return 2451545.0 + eph / 86400.0; }
+
+bool hip_ok(hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    fprintf(stderr, "(ASSIST) HIP error in %s: %s\n", what, hipGetErrorString(e));
+    return false;
+}
+
+// Flattens one SPK target into equal-length records [jd_ref-jul(MID), 86400/RADIUS, coeffs, pad].
+// Returns false when the target's segments are not contiguous with equal record length.
+bool flatten_target(const spk_s* s, const spk_target& t, double jd_ref, std::vector<double>& out, DevSlot& slot,
+                    size_t& offset_words) {
+    const double* base = (const double*)s->map;
+    double intlen = 0, init0 = 0;
+    int rsize = 0, nrec_total = 0;
+    for (size_t k = 0; k < t.two.size(); k++) {
+        const double* dir = base + t.two[k] - 1;  // -> N
+        const double init = dir[-3], len = dir[-2];
+        const int R = (int)dir[-1], N = (int)dir[0];
+        if (k == 0) { intlen = len; init0 = init; rsize = R; }
+        if (len != intlen || R != rsize || N <= 0) return false;
+        if (fabs(init - (init0 + (double)nrec_total * intlen)) > 1e-6 * intlen) return false;
+        const long need = (long)t.one[k] - 1 + (long)N * R;
+        if (need > (long)t.two[k]) return false;
+        nrec_total += N;
+    }
+    const int P = (rsize - 2) / 3;
+    if (P < 2 || P > kMaxCoef) return false;
+    const int Rd = ((2 + 3 * P) + 1) & ~1;  // even number of doubles
+    offset_words = out.size();
+    out.resize(out.size() + (size_t)nrec_total * Rd, 0.0);
+    double* dst = out.data() + offset_words;
+    for (size_t k = 0; k < t.two.size(); k++) {
+        const double* dir = base + t.two[k] - 1;
+        const int N = (int)dir[0];
+        const double* src = base + t.one[k] - 1;
+        for (int r = 0; r < N; r++) {
+            const double* rec = src + (size_t)r * rsize;
+            dst[0] = jd_ref - jul(rec[0]);
+            dst[1] = 1.0 / (rec[1] / 86400.0);
+            memcpy(dst + 2, rec + 2, sizeof(double) * 3 * P);
+            dst += Rd;
+        }
+    }
+    slot.rec = nullptr;
+    slot.c_init = jd_ref - jul(init0);
+    slot.inv_intlen = 1.0 / (intlen / 86400.0);
+    slot.nrec = nrec_total;
+    slot.P = P;
+    slot.R = Rd;
+    return true;
+}
+
+// Builds (or reuses) the device copy of the ephemeris on the current device.
+int get_device_ephem(const assist_ephem* e, const DevEphem** out, DevEphem* host_copy) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!e || !e->spk_planets) return ASSIST_ERROR_EPHEM_FILE;
+    if (e->ascii_planets) return ASSIST_BATCH_ERROR_UNSUPPORTED;
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return ASSIST_BATCH_ERROR_DEVICE;
+    spk_s* pl = e->spk_planets;
+    DeviceEphemCache* c = pl->gpu;
+    if (c && c->device == dev && c->jd_ref == e->jd_ref && c->asteroids == e->spk_asteroids) {
+        *out = c->d_ephem;
+        if (host_copy) *host_copy = c->host;
+        return ASSIST_SUCCESS;
+    }
+    if (c) { free_device_ephem(c); pl->gpu = nullptr; }
+    static const int planet_codes[12] = {10, 1, 2, 3, 399, 301, 4, 5, 6, 7, 8, 9};
+    DevEphem H;
+    memset(&H, 0, sizeof H);
+    std::vector<double> words;
+    size_t offs[kNumSlots];
+    const int n_ast = e->spk_asteroids ? e->spk_asteroids->num : 0;
+    if (n_ast > 16) return ASSIST_BATCH_ERROR_UNSUPPORTED;
+    for (int s = 0; s < kNumSlots; s++) {
+        const spk_s* file = s < kFirstAsteroidSlot ? pl : e->spk_asteroids;
+        int idx;
+        if (s < kFirstAsteroidSlot) idx = assist_host::find_target(pl, planet_codes[s]);
+        else idx = (s - kFirstAsteroidSlot) < n_ast ? s - kFirstAsteroidSlot : -1;
+        if (idx < 0) {
+            if (s < kFirstAsteroidSlot) return ASSIST_ERROR_NEPHEM;  // Earth/Moon/EMB fallbacks not on the GPU path
+            // missing asteroid slot: point it at the Sun's data (evaluated but never used)
+            H.slot[s] = H.slot[0];
+            offs[s] = offs[0];
+            H.slot[s].kind = 1;
+            continue;
+        }
+        if (!flatten_target(file, file->targets[idx], e->jd_ref, words, H.slot[s], offs[s])) return ASSIST_BATCH_ERROR_UNSUPPORTED;
+        H.slot[s].kind = s < kFirstAsteroidSlot ? 0 : 1;
+    }
+    static const int body_slot[kNumPlanets] = {0, 1, 2, 4, 5, 6, 7, 8, 9, 10, 11};
+    for (int b = 0; b < kNumBodies; b++) {
+        if (b < kNumPlanets) {
+            const int idx = assist_host::find_target(pl, planet_codes[body_slot[b]]);
+            H.GM[b] = idx >= 0 ? pl->targets[idx].mass : 0.0;
+        } else {
+            H.GM[b] = (b - kNumPlanets) < n_ast ? e->spk_asteroids->targets[b - kNumPlanets].mass : 0.0;
+        }
+    }
+    H.inv_au = 1.0 / e->AU;
+    H.inv_au_ast = 1.0 / 149597870.7;
+    H.vel_scale = 86400.0 / e->AU;
+    assist_ephem_time_bounds(e, &H.t_beg, &H.t_end);
+    H.n_ast = n_ast;
+    H.J2E = e->J2E; H.J3E = e->J3E; H.J4E = e->J4E; H.J2SUN = e->J2SUN;
+    H.Re_eq = e->Re_eq; H.Rs_eq = e->Rs_eq; H.c_squared = e->c_squared; H.over_c_squared = e->over_c_squared;
+    const double d2r = M_PI / 180.;
+    H.e_cosa = cos(0.0 * d2r); H.e_sina = sin(0.0 * d2r); H.e_cosd = cos(90.0 * d2r); H.e_sind = sin(90.0 * d2r);
+    H.s_cosa = cos(286.13 * d2r); H.s_sina = sin(286.13 * d2r); H.s_cosd = cos(63.87 * d2r); H.s_sind = sin(63.87 * d2r);
+    c = new DeviceEphemCache();
+    c->device = dev;
+    c->jd_ref = e->jd_ref;
+    c->asteroids = e->spk_asteroids;
+    if (!hip_ok(hipMalloc(&c->d_records, words.size() * sizeof(double)), "hipMalloc(ephemeris)") ||
+        !hip_ok(hipMemcpy(c->d_records, words.data(), words.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy(ephemeris)")) {
+        free_device_ephem(c);
+        return ASSIST_BATCH_ERROR_DEVICE;
+    }
+    for (int s = 0; s < kNumSlots; s++) H.slot[s].rec = c->d_records + offs[s];
+    if (!hip_ok(hipMalloc(&c->d_ephem, sizeof(DevEphem)), "hipMalloc(DevEphem)") ||
+        !hip_ok(hipMemcpy(c->d_ephem, &H, sizeof(DevEphem), hipMemcpyHostToDevice), "hipMemcpy(DevEphem)")) {
+        free_device_ephem(c);
+        return ASSIST_BATCH_ERROR_DEVICE;
+    }
+    c->host = H;
+    pl->gpu = c;
+    *out = c->d_ephem;
+    if (host_copy) *host_copy = H;
+    return ASSIST_SUCCESS;
+}
+
+int make_force_opts(const assist_extras* x, ForceOpts* fo) {
+    if (x) {
+        if (x->geocentric) return ASSIST_BATCH_ERROR_UNSUPPORTED;
+        if (x->gr_eih_sources > 1 && (x->forces & ASSIST_FORCE_GR_EIH)) return ASSIST_BATCH_ERROR_UNSUPPORTED;
+        fo->forces = x->forces;
+        fo->gr_eih_sources = x->gr_eih_sources;
+        fo->alpha = x->alpha; fo->nk = x->nk; fo->nm = x->nm; fo->nn = x->nn; fo->r0 = x->r0;
+    } else {
+        fo->forces = 0x7F;
+        fo->gr_eih_sources = 1;
+        fo->alpha = 1.0; fo->nk = 0.0; fo->nm = 2.0; fo->nn = 5.093; fo->r0 = 1.0;
+    }
+    return ASSIST_SUCCESS;
+}
+
+int lanes_for(int nvec, int requested) {
+    int g = 1;
+    while (g < nvec) g <<= 1;
+    if (requested > 0) {
+        if (requested != 1 && requested != 2 && requested != 4 && requested != 8) return -1;
+        if (requested < g) return -1;
+        return requested;
+    }
+    if (g > 8) return -1;
+    if (g < 8 && nvec == 1) g = 8;  // real-only systems: spread the 28 ephemeris targets over 8 lanes
+    return g;
+}
+
+// Persistent workspace on the current device (grown as needed; one launch at a time).
+struct Workspace {
+    int device = -1;
+    double* scratch = nullptr;
+    size_t scratch_words = 0;
+    int* queue = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+Workspace g_ws;
+
+int ensure_workspace(size_t scratch_words) {
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return ASSIST_BATCH_ERROR_DEVICE;
+    if (g_ws.device != dev) {
+        g_ws = Workspace();
+        g_ws.device = dev;
+        if (!hip_ok(hipMalloc(&g_ws.queue, sizeof(int)), "hipMalloc(queue)")) return ASSIST_BATCH_ERROR_DEVICE;
+        if (!hip_ok(hipEventCreate(&g_ws.ev0), "hipEventCreate") || !hip_ok(hipEventCreate(&g_ws.ev1), "hipEventCreate"))
+            return ASSIST_BATCH_ERROR_DEVICE;
+    }
+    if (scratch_words > g_ws.scratch_words) {
+        if (g_ws.scratch) (void)hipFree(g_ws.scratch);
+        g_ws.scratch = nullptr;
+        g_ws.scratch_words = 0;
+        if (!hip_ok(hipMalloc(&g_ws.scratch, scratch_words * sizeof(double)), "hipMalloc(scratch)")) return ASSIST_BATCH_ERROR_DEVICE;
+        g_ws.scratch_words = scratch_words;
+    }
+    return ASSIST_SUCCESS;
+}
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    bool alloc(size_t n) { return n == 0 || hip_ok(hipMalloc(&p, n * sizeof(T)), "hipMalloc"); }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// simulation hooks: every real particle with its variational particles is one system
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+struct SimLayout {
+    int n_real = 0, nvmax = 0;
+    std::vector<std::vector<int>> vars;  // per real particle: var-config indices
+};
+
+SimLayout layout_of(const reb_simulation* sim) {
+    SimLayout L;
+    L.n_real = sim->N - sim->N_var;
+    L.vars.assign(L.n_real > 0 ? L.n_real : 0, {});
+    for (int v = 0; v < sim->N_var_config; v++) {
+        const int tp = sim->var_config[v].testparticle;
+        if (tp >= 0 && tp < L.n_real) L.vars[tp].push_back(v);
+    }
+    for (auto& v : L.vars) L.nvmax = (int)v.size() > L.nvmax ? (int)v.size() : L.nvmax;
+    return L;
+}
+
+void pack(const reb_simulation* sim, const assist_extras* ax, const SimLayout& L, std::vector<double>& states,
+          std::vector<double>& params, bool& has_params) {
+    const int nv = 1 + L.nvmax;
+    states.assign((size_t)L.n_real * nv * 6, 0.0);
+    has_params = ax && ax->particle_params && (ax->forces & ASSIST_FORCE_NON_GRAVITATIONAL);
+    params.assign(has_params ? (size_t)L.n_real * nv * 3 : 0, 0.0);
+    for (int j = 0; j < L.n_real; j++) {
+        const reb_particle& p = sim->particles[j];
+        double* s = &states[(size_t)j * nv * 6];
+        s[0] = p.x; s[1] = p.y; s[2] = p.z; s[3] = p.vx; s[4] = p.vy; s[5] = p.vz;
+        if (has_params) for (int c = 0; c < 3; c++) params[(size_t)j * nv * 3 + c] = ax->particle_params[3 * j + c];
+        for (size_t k = 0; k < L.vars[j].size(); k++) {
+            const int vc = L.vars[j][k];
+            const reb_particle& q = sim->particles[sim->var_config[vc].index];
+            double* d = s + 6 * (k + 1);
+            d[0] = q.x; d[1] = q.y; d[2] = q.z; d[3] = q.vx; d[4] = q.vy; d[5] = q.vz;
+            if (has_params)
+                for (int c = 0; c < 3; c++) params[((size_t)j * nv + k + 1) * 3 + c] = ax->particle_params[3 * (L.n_real + vc) + c];
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void assist_batch_config_default(struct assist_batch_config* cfg) {
+    cfg->epsilon = 1e-9;
+    cfg->min_dt = 0.0;
+    cfg->dt_initial = 0.001;
+    cfg->max_steps = 1000000LL;
+    cfg->precision = ASSIST_PRECISION_FP64;
+    cfg->lanes_per_system = 0;
+    cfg->block_size = 0;
+    cfg->blocks_per_cu = 0;
+}
+
+const char* assist_batch_error_string(int code) {
+    if (code >= 0 && code < ASSIST_ERROR_N) return assist_error_messages[code];
+    switch (code) {
+    case ASSIST_BATCH_ERROR_ARG: return "Invalid argument.";
+    case ASSIST_BATCH_ERROR_DEVICE: return "GPU runtime error (no device, allocation or launch failure).";
+    case ASSIST_BATCH_ERROR_UNSUPPORTED: return "Option not supported by the GPU path.";
+    default: return "Unknown error.";
+    }
+}
+
+int assist_gpu_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void* assist_device_malloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    return p;
+}
+void assist_device_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+int assist_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    return hip_ok(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy") ? 0 : ASSIST_BATCH_ERROR_DEVICE;
+}
+int assist_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    return hip_ok(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy") ? 0 : ASSIST_BATCH_ERROR_DEVICE;
+}
+int assist_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+    return hip_ok(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy") ? 0 : ASSIST_BATCH_ERROR_DEVICE;
+}
+int assist_device_synchronize(void) {
+    return hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize") ? 0 : ASSIST_BATCH_ERROR_DEVICE;
+}
+
+int assist_batch_forces(const struct assist_ephem* ephem, const struct assist_extras* opts, int n, int n_var,
+                        const double* t, const double* states, const double* params, double* acc, int* status,
+                        int precision) {
+    if (n < 0 || n_var < 0 || n_var > 7 || !t || !states || !acc) return ASSIST_BATCH_ERROR_ARG;
+    if (n == 0) return ASSIST_SUCCESS;
+    if (assist_gpu_device_count() <= 0) return ASSIST_BATCH_ERROR_DEVICE;
+    ForcesArgs a;
+    memset(&a, 0, sizeof a);
+    int rc = make_force_opts(opts, &a.fo);
+    if (rc) return rc;
+    rc = get_device_ephem(ephem, &a.E, nullptr);
+    if (rc) return rc;
+    const int nv = 1 + n_var;
+    const int G = lanes_for(nv, 0);
+    if (G < 0) return ASSIST_BATCH_ERROR_ARG;
+    DevBuf<double> dt, ds, dp, da;
+    DevBuf<int> dst;
+    const size_t ns = (size_t)n;
+    if (!dt.alloc(ns) || !ds.alloc(ns * nv * 6) || !da.alloc(ns * nv * 3) || !dst.alloc(ns) ||
+        (params && !dp.alloc(ns * nv * 3)))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    if (!hip_ok(hipMemcpy(dt.p, t, ns * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy") ||
+        !hip_ok(hipMemcpy(ds.p, states, ns * nv * 6 * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy") ||
+        (params && !hip_ok(hipMemcpy(dp.p, params, ns * nv * 3 * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    a.nsys = n;
+    a.nvec = nv;
+    a.t = dt.p;
+    a.states = ds.p;
+    a.params = params ? dp.p : nullptr;
+    a.acc = da.p;
+    a.status = dst.p;
+    if (!hip_ok(launch_forces(a, G, precision == ASSIST_PRECISION_MIXED, nullptr), "launch(batch_forces)") ||
+        !hip_ok(hipDeviceSynchronize(), "batch_forces"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    if (!hip_ok(hipMemcpy(acc, da.p, ns * nv * 3 * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy") ||
+        (status && !hip_ok(hipMemcpy(status, dst.p, ns * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy")))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    return ASSIST_SUCCESS;
+}
+
+int assist_batch_integrate_device(const struct assist_ephem* ephem, const struct assist_extras* opts,
+                                  const struct assist_batch_config* cfg_in, int n, int n_var, const double* d_t0,
+                                  const double* d_t1, double* d_states, const double* d_params, int* d_status,
+                                  long long* d_steps, long long* d_evals, void* stream,
+                                  struct assist_batch_stats* stats) {
+    if (n < 0 || n_var < 0 || n_var > 7 || !d_t0 || !d_t1 || !d_states || !d_status) return ASSIST_BATCH_ERROR_ARG;
+    struct assist_batch_config cfg;
+    if (cfg_in) cfg = *cfg_in;
+    else assist_batch_config_default(&cfg);
+    if (n == 0) return ASSIST_SUCCESS;
+    IntegrateArgs a;
+    memset(&a, 0, sizeof a);
+    int rc = make_force_opts(opts, &a.fo);
+    if (rc) return rc;
+    rc = get_device_ephem(ephem, &a.E, nullptr);
+    if (rc) return rc;
+    const int nv = 1 + n_var;
+    const int G = lanes_for(nv, cfg.lanes_per_system);
+    if (G < 0) return ASSIST_BATCH_ERROR_ARG;
+    const bool mixed = cfg.precision == ASSIST_PRECISION_MIXED;
+    const int threads = cfg.block_size > 0 ? cfg.block_size : 256;
+    if (threads % 64 != 0 || threads > 256) return ASSIST_BATCH_ERROR_ARG;
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice") ||
+        !hip_ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute") ||
+        !hip_ok(ias15_occupancy(G, mixed, threads, &per_cu), "occupancy"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    if (per_cu < 1) per_cu = 1;
+    if (cfg.blocks_per_cu > 0 && cfg.blocks_per_cu < per_cu) per_cu = cfg.blocks_per_cu;
+    const long long need_blocks = ((long long)n * G + threads - 1) / threads;
+    const long long max_blocks = (long long)per_cu * ncu;
+    const int blocks = (int)(need_blocks < max_blocks ? need_blocks : max_blocks);
+    const int total_groups = blocks * threads / G;
+    rc = ensure_workspace((size_t)blocks * threads * 42);
+    if (rc) return rc;
+    a.io.epsilon = cfg.epsilon;
+    a.io.min_dt = cfg.min_dt;
+    a.io.dt_initial = cfg.dt_initial;
+    a.io.max_steps = cfg.max_steps;
+    a.nsys = n;
+    a.nvec = nv;
+    a.t0 = d_t0;
+    a.t1 = d_t1;
+    a.states = d_states;
+    a.params = d_params;
+    a.status = d_status;
+    a.steps = d_steps;
+    a.evals = d_evals;
+    a.scratch = g_ws.scratch;
+    a.queue = g_ws.queue;
+    hipStream_t s = (hipStream_t)stream;
+    if (!hip_ok(launch_set_counter(g_ws.queue, total_groups, s), "launch(set_counter)") ||
+        !hip_ok(hipEventRecord(g_ws.ev0, s), "hipEventRecord") ||
+        !hip_ok(launch_ias15(a, G, mixed, blocks, threads, s), "launch(ias15)") ||
+        !hip_ok(hipEventRecord(g_ws.ev1, s), "hipEventRecord") ||
+        !hip_ok(hipEventSynchronize(g_ws.ev1), "ias15"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    if (stats) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, g_ws.ev0, g_ws.ev1);
+        stats->kernel_ms = ms;
+        stats->lanes_per_system = G;
+        stats->block_size = threads;
+        stats->grid_blocks = blocks;
+    }
+    return ASSIST_SUCCESS;
+}
+
+int assist_batch_integrate(const struct assist_ephem* ephem, const struct assist_extras* opts,
+                           const struct assist_batch_config* cfg, int n, int n_var, const double* t0,
+                           const double* t1, double* states, const double* params, int* status, long long* steps,
+                           struct assist_batch_stats* stats) {
+    if (n < 0 || n_var < 0 || n_var > 7 || !t0 || !t1 || !states) return ASSIST_BATCH_ERROR_ARG;
+    if (n == 0) return ASSIST_SUCCESS;
+    if (assist_gpu_device_count() <= 0) return ASSIST_BATCH_ERROR_DEVICE;
+    const int nv = 1 + n_var;
+    const size_t ns = (size_t)n;
+    DevBuf<double> d0, d1, ds, dp;
+    DevBuf<int> dst;
+    DevBuf<long long> dsteps, devals, drej;
+    if (!d0.alloc(ns) || !d1.alloc(ns) || !ds.alloc(ns * nv * 6) || !dst.alloc(ns) || !dsteps.alloc(ns) ||
+        !devals.alloc(ns) || (params && !dp.alloc(ns * nv * 3)))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    if (!hip_ok(hipMemcpy(d0.p, t0, ns * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy") ||
+        !hip_ok(hipMemcpy(d1.p, t1, ns * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy") ||
+        !hip_ok(hipMemcpy(ds.p, states, ns * nv * 6 * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy") ||
+        (params && !hip_ok(hipMemcpy(dp.p, params, ns * nv * 3 * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    struct assist_batch_stats st;
+    memset(&st, 0, sizeof st);
+    int rc = assist_batch_integrate_device(ephem, opts, cfg, n, n_var, d0.p, d1.p, ds.p, params ? dp.p : nullptr,
+                                           dst.p, dsteps.p, devals.p, nullptr, &st);
+    if (rc) return rc;
+    std::vector<long long> hsteps(ns), hevals(ns);
+    if (!hip_ok(hipMemcpy(states, ds.p, ns * nv * 6 * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy") ||
+        !hip_ok(hipMemcpy(hsteps.data(), dsteps.p, ns * sizeof(long long), hipMemcpyDeviceToHost), "hipMemcpy") ||
+        !hip_ok(hipMemcpy(hevals.data(), devals.p, ns * sizeof(long long), hipMemcpyDeviceToHost), "hipMemcpy") ||
+        (status && !hip_ok(hipMemcpy(status, dst.p, ns * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy")))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    if (steps) memcpy(steps, hsteps.data(), ns * sizeof(long long));
+    for (size_t i = 0; i < ns; i++) { st.steps += hsteps[i]; st.evals += hevals[i]; }
+    if (stats) *stats = st;
+    return ASSIST_SUCCESS;
+}
+
+void assist_additional_forces(struct reb_simulation* sim) {
+    auto* ax = (struct assist_extras*)sim->extras;
+    if (!ax || !ax->ephem) return;
+    SimLayout L = layout_of(sim);
+    if (L.n_real <= 0) return;
+    std::vector<double> states, params, acc;
+    bool hp = false;
+    pack(sim, ax, L, states, params, hp);
+    const int nv = 1 + L.nvmax;
+    std::vector<double> t(L.n_real, sim->t);
+    std::vector<int> st(L.n_real, 0);
+    acc.assign((size_t)L.n_real * nv * 3, 0.0);
+    const int rc = assist_batch_forces(ax->ephem, ax, L.n_real, L.nvmax, t.data(), states.data(), hp ? params.data() : nullptr,
+                                       acc.data(), st.data(), ASSIST_PRECISION_FP64);
+    if (rc) {
+        reb_simulation_error(sim, assist_batch_error_string(rc));
+        return;
+    }
+    for (int j = 0; j < L.n_real; j++) {
+        if (st[j]) {
+            reb_simulation_error(sim, assist_error_messages[st[j] < ASSIST_ERROR_N ? st[j] : 0]);
+            return;
+        }
+        reb_particle& p = sim->particles[j];
+        const double* a = &acc[(size_t)j * nv * 3];
+        p.ax = a[0]; p.ay = a[1]; p.az = a[2];
+        for (size_t k = 0; k < L.vars[j].size(); k++) {
+            reb_particle& q = sim->particles[sim->var_config[L.vars[j][k]].index];
+            q.ax = a[3 * (k + 1)]; q.ay = a[3 * (k + 1) + 1]; q.az = a[3 * (k + 1) + 2];
+        }
+    }
+}
+
+void reb_simulation_integrate(struct reb_simulation* sim, double tmax) {
+    auto* ax = (struct assist_extras*)sim->extras;
+    if (!ax || !ax->ephem) {
+        reb_simulation_error(sim, "No ASSIST extras attached to the simulation.");
+        return;
+    }
+    SimLayout L = layout_of(sim);
+    if (L.n_real <= 0) { sim->t = tmax; return; }
+    std::vector<double> states, params;
+    bool hp = false;
+    pack(sim, ax, L, states, params, hp);
+    const int nv = 1 + L.nvmax;
+    std::vector<double> t0(L.n_real, sim->t), t1(L.n_real, tmax);
+    std::vector<int> st(L.n_real, 0);
+    std::vector<long long> steps(L.n_real, 0);
+    struct assist_batch_config cfg;
+    assist_batch_config_default(&cfg);
+    cfg.epsilon = sim->ri_ias15.epsilon;
+    cfg.min_dt = sim->ri_ias15.min_dt;
+    if (sim->dt != 0) cfg.dt_initial = fabs(sim->dt);
+    const int rc = assist_batch_integrate(ax->ephem, ax, &cfg, L.n_real, L.nvmax, t0.data(), t1.data(), states.data(),
+                                          hp ? params.data() : nullptr, st.data(), steps.data(), nullptr);
+    if (rc) {
+        reb_simulation_error(sim, assist_batch_error_string(rc));
+        return;
+    }
+    for (int j = 0; j < L.n_real; j++) {
+        const double* s = &states[(size_t)j * nv * 6];
+        reb_particle& p = sim->particles[j];
+        p.x = s[0]; p.y = s[1]; p.z = s[2]; p.vx = s[3]; p.vy = s[4]; p.vz = s[5];
+        for (size_t k = 0; k < L.vars[j].size(); k++) {
+            reb_particle& q = sim->particles[sim->var_config[L.vars[j][k]].index];
+            const double* d = s + 6 * (k + 1);
+            q.x = d[0]; q.y = d[1]; q.z = d[2]; q.vx = d[3]; q.vy = d[4]; q.vz = d[5];
+        }
+        ax->steps_done += (int)steps[j];
+        if (st[j] != 0) {
+            reb_simulation_error(sim, st[j] > 0 && st[j] < ASSIST_ERROR_N ? assist_error_messages[st[j]]
+                                                                        : "Integration of a particle did not finish.");
+        }
+    }
+    if (!sim->status) sim->t = tmax;
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// Device-side layout of the ephemeris and per-launch arguments.
+//
+// The SPK type-2 data of the 28 Chebyshev targets the force model needs are
+// flattened into one device buffer.  Each target ("slot") is a contiguous run
+// of equal-length records; a record holds
+//     [ jd_ref - jul(MID),  86400/RADIUS,  x coeffs (P), y coeffs (P), z coeffs (P), pad ]
+// padded to an even number of doubles so 16-byte vector loads stay aligned.
+// The first two words replace the SPK record's MID and RADIUS with the
+// quantities the evaluation actually uses (reference src/spk.c:445,514:
+// z = ((jd_ref - jul(MID)) + t) / (RADIUS/86400)).
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define ASSIST_HD __host__ __device__
+#else
+#define ASSIST_HD
+#endif
+
+namespace assist_gpu {
+
+constexpr int kNumSlots = 28;       // 12 planet-file targets + 16 asteroids
+constexpr int kNumBodies = 27;      // ASSIST body indices 0..26
+constexpr int kNumPlanets = 11;     // ASSIST_BODY_NPLANETS
+constexpr int kMaxCoef = 16;
+constexpr int kSlotEMB = 3;         // Earth-Moon barycentre (no force of its own)
+constexpr int kSlotEarth = 4;       // Earth relative to the EMB
+constexpr int kSlotMoon = 5;        // Moon relative to the EMB
+constexpr int kFirstAsteroidSlot = 12;
+
+// Slot order: Sun, Mercury, Venus, EMB, Earth/EMB, Moon/EMB, Mars, Jupiter,
+// Saturn, Uranus, Neptune, Pluto, asteroids 0..15.  slot -> ASSIST body index
+// (-1 for the EMB).
+ASSIST_HD constexpr int slot_body(int s) {
+    return s < 3 ? s : (s == 3 ? -1 : s - 1);
+}
+
+struct DevSlot {
+    const double* rec;   // first record
+    double c_init;       // jd_ref - jul(INIT) of the first record
+    double inv_intlen;   // 86400 / INTLEN  (1/record length in days)
+    int nrec;
+    int P;               // coefficients per coordinate
+    int R;               // record stride in doubles (padded)
+    int kind;            // 0 planet-file target, 1 asteroid
+};
+
+struct DevEphem {
+    DevSlot slot[kNumSlots];
+    double GM[kNumBodies];
+    double inv_au;        // 1 / AU [km]              (planets, reference src/spk.c:590)
+    double inv_au_ast;    // 1 / 149597870.7          (asteroids, reference src/spk.c:470)
+    double vel_scale;     // 86400 / AU               (km/s -> au/d)
+    double t_beg, t_end;  // common coverage, days relative to jd_ref
+    int n_ast;            // asteroids present (0 if no asteroid file)
+    int pad_;
+    // force-model constants (reference src/spk.c:769-789)
+    double J2E, J3E, J4E, J2SUN, Re_eq, Rs_eq, c_squared, over_c_squared;
+    // pole rotations: Earth (RA 0, Dec 90 deg) and Sun (RA 286.13, Dec 63.87 deg),
+    // cos/sin computed on the host (reference src/forces.c:484-490,670-676)
+    double e_cosa, e_sina, e_cosd, e_sind;
+    double s_cosa, s_sina, s_cosd, s_sind;
+};
+
+struct ForceOpts {
+    int forces;          // ASSIST_FORCE_* bitmask
+    int gr_eih_sources;  // 0 or 1 on the device
+    double alpha, nk, nm, nn, r0;  // Marsden g(r) shape
+};
+
+// IAS15 configuration for the batched integrator.
+struct IntegratorOpts {
+    double epsilon;
+    double min_dt;
+    double dt_initial;
+    long long max_steps;
+};
+
+enum SystemStatus : int {
+    kOk = 0,
+    kCoverage = 5,        // ASSIST_ERROR_COVERAGE
+    kNonFinite = -1,
+    kMaxSteps = -2,
+    kDtUnderflow = -3,
+};
+
+}  // namespace assist_gpu

@@ -1,0 +1,669 @@
+// The ASSIST force model on the GPU, evaluated cooperatively by a group of G
+// lanes (G = 1, 2, 4 or 8) that all work on one system (one real particle and
+// its variational particles) at one time t.
+//
+//   1. Chebyshev evaluation of the 28 ephemeris targets, slots spread over the
+//      G lanes (reference src/spk.c:405-547, 550-610).
+//   2. Composition: Earth and Moon = EMB + offset, asteroids = heliocentric +
+//      Sun (reference src/spk.c:572-587, src/forces.c:213-219).
+//   3. Per-body work on the lane that evaluated the body: point-mass
+//      acceleration, its Jacobian, and the EIH sums over the planets
+//      (reference src/forces.c:266-433, 1319-1501).
+//   4. A butterfly sum over the group, after which every lane holds identical
+//      totals.
+//   5. The single-particle terms (non-gravitational, Earth J2/J3/J4, solar J2,
+//      EIH, simple/potential GR), evaluated redundantly on every lane (the
+//      lanes of a wave execute one instruction stream, so splitting them over
+//      lanes would serialise anyway) (reference src/forces.c:435-1983).
+//
+// The variational equations are linear in the variational state, so instead
+// of applying each term's partials to each variational particle the way the
+// reference does, the terms add their partials into one 3x9 Jacobian
+// [d a/d x | d a/d v | d a/d A] that the variational lanes apply once.  The
+// real particle's acceleration is summed in the reference's term order
+// (NG, Earth harmonics, solar J2, EIH, GR potential, GR simple, direct).
+//
+// Mixed precision (Mixed = true): Chebyshev evaluation, the Sun/planet/Earth/
+// Moon point masses and everything in IAS15 stay in FP64; the 16 asteroid
+// point masses, J3/J4, GR, non-gravitational forces and every Jacobian run in
+// FP32.
+#pragma once
+#include <math.h>
+
+#include <type_traits>
+
+#include "device_types.h"
+
+// Host-and-device qualifier: the kernels use these functions on the GPU; the
+// operation-count tool instantiates them on the host with a counting scalar.
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define AHD __host__ __device__ __forceinline__
+#else
+#define AHD inline
+#endif
+
+namespace assist_gpu {
+
+enum : int {
+    F_SUN = 0x01, F_PLANETS = 0x02, F_ASTEROIDS = 0x04, F_NONGRAV = 0x08, F_EARTH_H = 0x10,
+    F_SUN_H = 0x20, F_GR_EIH = 0x40, F_GR_SIMPLE = 0x80, F_GR_POTENTIAL = 0x100
+};
+
+// ---- lane-group collectives ----------------------------------------------------------
+// Butterfly collectives over the G lanes of a group (G = 1: identity).
+template <int G, typename T>
+AHD T gsum(T v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) v += __shfl_xor(v, m, G);
+#endif
+    return v;
+}
+template <int G>
+AHD double gmax(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) v = fmax(v, __shfl_xor(v, m, G));
+#endif
+    return v;
+}
+template <int G, typename T>
+AHD T gbcast(T v, int src) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (G > 1) return __shfl(v, src, G);
+#endif
+    (void)src;
+    return v;
+}
+template <int G>
+AHD int gbcast_i(int v, int src) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (G > 1) return __shfl(v, src, G);
+#endif
+    (void)src;
+    return v;
+}
+template <int G>
+AHD int gor(int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) v |= __shfl_xor(v, m, G);
+#endif
+    return v;
+}
+
+// Single-precision partner of a double-precision scalar type (float for double).
+template <typename D>
+struct SingleOf { using type = float; };
+
+AHD double value_of(double x) { return x; }
+AHD float value_of(float x) { return x; }
+
+AHD bool is_normal(double x) {
+    const double a = fabs(x);
+    return a >= 2.2250738585072014e-308 && a <= 1.7976931348623157e308;
+}
+
+// ---- Chebyshev evaluation -------------------------------------------------------------
+// Record selection follows reference src/spk.c:440-445 (integer truncation of the
+// record offset, then z in [-1, 1]); indices are clamped so a time at the very end of
+// coverage reads the last record instead of the directory words.
+template <typename D>
+struct ChebOut {
+    D x, y, z;     // km
+    D vx, vy, vz;  // km/day (only when requested)
+};
+
+template <typename D>
+AHD ChebOut<D> cheb_eval(const DevSlot& S, D t, bool want_vel) {
+    const D fi = (D(S.c_init) + t) * D(S.inv_intlen);
+    int idx = (int)value_of(fi);
+    idx = idx < 0 ? 0 : (idx >= S.nrec ? S.nrec - 1 : idx);
+    const double* rec = S.rec + (size_t)idx * S.R;
+    const D z = (D(rec[0]) + t) * D(rec[1]);
+    const int P = S.P;
+    const double* cx = rec + 2;
+    const double* cy = cx + P;
+    const double* cz = cy + P;
+    const D z2 = D(2.0) * z;
+    D T0 = D(1.0), T1 = z;
+    D x = fma(D(cx[1]), z, D(cx[0]));
+    D y = fma(D(cy[1]), z, D(cy[0]));
+    D w = fma(D(cz[1]), z, D(cz[0]));
+    ChebOut<D> o;
+    if (!want_vel) {
+        for (int p = 2; p < P; p++) {
+            const D T2 = fma(z2, T1, -T0);
+            x = fma(D(cx[p]), T2, x);
+            y = fma(D(cy[p]), T2, y);
+            w = fma(D(cz[p]), T2, w);
+            T0 = T1;
+            T1 = T2;
+        }
+        o.vx = o.vy = o.vz = D(0.0);
+    } else {
+        D S0 = D(0.0), S1 = D(1.0);
+        D vx = D(cx[1]), vy = D(cy[1]), vz = D(cz[1]);
+        for (int p = 2; p < P; p++) {
+            const D T2 = fma(z2, T1, -T0);
+            const D S2 = fma(z2, S1, fma(D(2.0), T1, -S0));
+            x = fma(D(cx[p]), T2, x);
+            y = fma(D(cy[p]), T2, y);
+            w = fma(D(cz[p]), T2, w);
+            vx = fma(D(cx[p]), S2, vx);
+            vy = fma(D(cy[p]), S2, vy);
+            vz = fma(D(cz[p]), S2, vz);
+            T0 = T1;
+            T1 = T2;
+            S0 = S1;
+            S1 = S2;
+        }
+        o.vx = vx * D(rec[1]);
+        o.vy = vy * D(rec[1]);
+        o.vz = vz * D(rec[1]);
+    }
+    o.x = x;
+    o.y = y;
+    o.z = w;
+    return o;
+}
+
+// ---- the 3x9 Jacobian of the real acceleration --------------------------------------------
+template <typename T>
+struct Jac {
+    T x[3][3];  // d a_i / d x_j
+    T v[3][3];  // d a_i / d v_j
+    T A[3][3];  // d a_i / d A_j (Marsden parameters)
+};
+
+template <typename T>
+AHD void jac_zero(Jac<T>& J) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) J.x[i][j] = J.v[i][j] = J.A[i][j] = T(0);
+}
+
+// Rotation into a body's equatorial frame (rows of R), written out as in
+// reference src/forces.c:503-505 / 542-544.
+template <typename T>
+struct Rot {
+    T r[3][3];
+    AHD Rot(double cosa, double sina, double cosd, double sind) {
+        r[0][0] = T(-sina); r[0][1] = T(cosa); r[0][2] = T(0);
+        r[1][0] = T(-cosa * sind); r[1][1] = T(-sina * sind); r[1][2] = T(cosd);
+        r[2][0] = T(cosa * cosd); r[2][1] = T(sina * cosd); r[2][2] = T(sind);
+    }
+};
+
+// J += R^T S R for a symmetric S given as (xx, yy, zz, xy, yz, xz)
+template <typename T>
+AHD void add_rotated_sym(T J[3][3], const Rot<T>& R, T sxx, T syy, T szz, T sxy, T syz, T sxz) {
+    const T S[3][3] = {{sxx, sxy, sxz}, {sxy, syy, syz}, {sxz, syz, szz}};
+    T SR[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) SR[i][j] = S[i][0] * R.r[0][j] + S[i][1] * R.r[1][j] + S[i][2] * R.r[2][j];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) J[i][j] += R.r[0][i] * SR[0][j] + R.r[1][i] * SR[1][j] + R.r[2][i] * SR[2][j];
+}
+
+// ---- single-particle terms ---------------------------------------------------------------
+
+// Earth J2/J3/J4 (reference src/forces.c:435-642).  J2 in D; J3/J4 and the partials in TS.
+template <typename D, typename TS>
+AHD void term_earth_J2J4(const DevEphem& E, const D xr[3], const D xe[3], D GM, bool need_jac, D acc[3],
+                         Jac<TS>& Jt) {
+    const D cosa = D(E.e_cosa), sina = D(E.e_sina), cosd = D(E.e_cosd), sind = D(E.e_sind);
+    D dx = xr[0] - xe[0], dy = xr[1] - xe[1], dz = xr[2] - xe[2];
+    const D r2 = dx * dx + dy * dy + dz * dz;
+    const D r = sqrt(r2);
+    const D dxp = -dx * sina + dy * cosa;
+    const D dyp = -dx * cosa * sind - dy * sina * sind + dz * cosd;
+    const D dzp = dx * cosa * cosd + dy * sina * cosd + dz * sind;
+    dx = dxp; dy = dyp; dz = dzp;
+    const D Re = D(E.Re_eq);
+    const D costheta2 = dz * dz / r2;
+    const D J2p = D(3.) * D(E.J2E) * Re * Re / r2 / r2 / r / D(2.);
+    const D J2f = D(5.) * costheta2 - D(1.);
+    D rx = GM * J2p * J2f * dx;
+    D ry = GM * J2p * J2f * dy;
+    D rz = GM * J2p * (J2f - D(2.)) * dz;
+    const TS ct2 = TS(costheta2), tdx = TS(dx), tdy = TS(dy), tdz = TS(dz), tr2 = TS(r2), tr = TS(r), tGM = TS(GM);
+    const TS tRe = TS(Re);
+    const TS J3p = TS(5.) * TS(E.J3E) * tRe * tRe * tRe / tr2 / tr2 / tr / TS(2.);
+    const TS J3f = TS(3.) - TS(7.) * ct2;
+    const TS J4p = TS(5.) * TS(E.J4E) * tRe * tRe * tRe * tRe / tr2 / tr2 / tr2 / tr / TS(8.);
+    const TS J4f = TS(63.) * ct2 * ct2 - TS(42.) * ct2 + TS(3.);
+    const TS j3x = -tGM * J3p * (TS(1.) / tr2) * J3f * tdx * tdz;
+    const TS j3y = -tGM * J3p * (TS(1.) / tr2) * J3f * tdy * tdz;
+    const TS j3z = -tGM * J3p * (TS(6.) * ct2 - TS(7.) * ct2 * ct2 - TS(0.6));
+    const TS j4x = tGM * J4p * J4f * tdx;
+    const TS j4y = tGM * J4p * J4f * tdy;
+    const TS j4z = tGM * J4p * (J4f + TS(12.) - TS(28.) * ct2) * tdz;
+    rx += D(j3x); ry += D(j3y); rz += D(j3z);
+    rx += D(j4x); ry += D(j4y); rz += D(j4z);
+    acc[0] += -rx * sina - ry * cosa * sind + rz * cosa * cosd;
+    acc[1] += rx * cosa - ry * sina * sind + rz * sina * cosd;
+    acc[2] += +ry * cosd + rz * sind;
+    if (!need_jac) return;
+    const TS J2pT = TS(J2p), J2fT = TS(J2f);
+    const TS f2 = TS(7.) * ct2 - TS(1.);
+    const TS f3 = TS(35.) * ct2 * ct2 - TS(30.) * ct2 + TS(3.);
+    TS sxx = tGM * J2pT * (J2fT - TS(5.) * f2 * tdx * tdx / tr2);
+    TS syy = tGM * J2pT * (J2fT - TS(5.) * f2 * tdy * tdy / tr2);
+    TS szz = tGM * J2pT * TS(-1.) * f3;
+    TS sxy = tGM * J2pT * TS(-5.) * f2 * tdx * tdy / tr2;
+    TS syz = tGM * J2pT * TS(-5.) * (f2 - TS(2.)) * tdy * tdz / tr2;
+    TS sxz = tGM * J2pT * TS(-5.) * (f2 - TS(2.)) * tdx * tdz / tr2;
+    const TS ct = tdz / tr;
+    const TS g2 = TS(21) * (TS(-3.) * ct2 + TS(1.)) / tr2;
+    const TS g3 = TS(3) * (TS(-21.) * ct2 * ct2 + TS(14.) * ct2 - TS(1.)) / tr2;
+    const TS g4 = (TS(-63.) * ct2 * ct2 + TS(70.) * ct2 - TS(15.)) * ct / tr;
+    sxx += tGM * J3p * ct * (g2 * tdx * tdx - J3f) / tr;
+    syy += tGM * J3p * ct * (g2 * tdy * tdy - J3f) / tr;
+    szz += tGM * J3p * g4;
+    sxy += tGM * J3p * g2 * ct * tdx * tdy / tr;
+    syz += tGM * J3p * g3 * tdy;
+    sxz += tGM * J3p * g3 * tdx;
+    const TS h2 = TS(33.) * ct2 * ct2 - TS(18.) * ct2 + TS(1.);
+    const TS h3 = TS(33.) * ct2 * ct2 - TS(30.) * ct2 + TS(5.);
+    const TS h4 = TS(231.) * ct2 * ct2 * ct2 - TS(315.) * ct2 * ct2 + TS(105.) * ct2 - TS(5.);
+    sxx += tGM * J4p * (J4f - TS(21.) * h2 * tdx * tdx / tr2);
+    syy += tGM * J4p * (J4f - TS(21.) * h2 * tdy * tdy / tr2);
+    szz += tGM * J4p * TS(-3.) * h4;
+    sxy += tGM * J4p * TS(-21.) * h2 * tdx * tdy / tr2;
+    syz += tGM * J4p * TS(-21.) * h3 * tdy * tdz / tr2;
+    sxz += tGM * J4p * TS(-21.) * h3 * tdx * tdz / tr2;
+    Rot<TS> R(E.e_cosa, E.e_sina, E.e_cosd, E.e_sind);
+    add_rotated_sym(Jt.x, R, sxx, syy, szz, sxy, syz, sxz);
+}
+
+// Solar J2 (reference src/forces.c:644-772), in D; partials in TS.
+template <typename D, typename TS>
+AHD void term_solar_J2(const DevEphem& E, const D xr[3], const D xs[3], D GMs, bool need_jac, D acc[3], Jac<TS>& Jt) {
+    const D cosa = D(E.s_cosa), sina = D(E.s_sina), cosd = D(E.s_cosd), sind = D(E.s_sind);
+    D dx = xr[0] - xs[0], dy = xr[1] - xs[1], dz = xr[2] - xs[2];
+    const D r2 = dx * dx + dy * dy + dz * dz;
+    const D r = sqrt(r2);
+    const D dxp = -dx * sina + dy * cosa;
+    const D dyp = -dx * cosa * sind - dy * sina * sind + dz * cosd;
+    const D dzp = dx * cosa * cosd + dy * sina * cosd + dz * sind;
+    dx = dxp; dy = dyp; dz = dzp;
+    const D costheta2 = dz * dz / r2;
+    const D pf = D(3.) * D(E.J2SUN) * D(E.Rs_eq) * D(E.Rs_eq) / r2 / r2 / r / D(2.);
+    const D f1 = D(5.) * costheta2 - D(1.);
+    const D rx = GMs * pf * f1 * dx;
+    const D ry = GMs * pf * f1 * dy;
+    const D rz = GMs * pf * (f1 - D(2.)) * dz;
+    acc[0] += -rx * sina - ry * cosa * sind + rz * cosa * cosd;
+    acc[1] += rx * cosa - ry * sina * sind + rz * sina * cosd;
+    acc[2] += +ry * cosd + rz * sind;
+    if (!need_jac) return;
+    const TS t2 = TS(costheta2), tdx = TS(dx), tdy = TS(dy), tdz = TS(dz), tr2 = TS(r2);
+    const TS gp = TS(GMs * pf), tf1 = TS(f1);
+    const TS f2 = TS(7.) * t2 - TS(1.);
+    const TS f3 = TS(35.) * t2 * t2 - TS(30.) * t2 + TS(3.);
+    const TS sxx = gp * (tf1 - TS(5.) * f2 * tdx * tdx / tr2);
+    const TS syy = gp * (tf1 - TS(5.) * f2 * tdy * tdy / tr2);
+    const TS szz = gp * TS(-1.) * f3;
+    const TS sxy = gp * TS(-5.) * f2 * tdx * tdy / tr2;
+    const TS syz = gp * TS(-5.) * (f2 - TS(2.)) * tdy * tdz / tr2;
+    const TS sxz = gp * TS(-5.) * (f2 - TS(2.)) * tdx * tdz / tr2;
+    Rot<TS> R(E.s_cosa, E.s_sina, E.s_cosd, E.s_sind);
+    add_rotated_sym(Jt.x, R, sxx, syy, szz, sxy, syz, sxz);
+}
+
+// Marsden non-gravitational acceleration (reference src/forces.c:774-1057), in TS.
+template <typename D, typename TS>
+AHD void term_nongrav(const ForceOpts& O, const D xr[3], const D vr[3], const D xs[3], const D vs[3], const double A[3],
+                      bool need_jac, D acc[3], Jac<TS>& J) {
+    const TS dx = TS(xr[0] - xs[0]), dy = TS(xr[1] - xs[1]), dz = TS(xr[2] - xs[2]);
+    const TS dvx = TS(vr[0] - vs[0]), dvy = TS(vr[1] - vs[1]), dvz = TS(vr[2] - vs[2]);
+    const TS A1 = TS(A[0]), A2 = TS(A[1]), A3 = TS(A[2]);
+    const TS alpha = TS(O.alpha), nk = TS(O.nk), nm = TS(O.nm), nn = TS(O.nn), r0 = TS(O.r0);
+    const TS r2 = dx * dx + dy * dy + dz * dz;
+    const TS r = sqrt(r2);
+    const TS g = alpha * pow(r / r0, -nm) * pow(TS(1.0) + pow(r / r0, nn), -nk);
+    const TS hx = dy * dvz - dz * dvy, hy = dz * dvx - dx * dvz, hz = dx * dvy - dy * dvx;
+    const TS h = sqrt(hx * hx + hy * hy + hz * hz);
+    const TS tx = hy * dz - hz * dy, ty = hz * dx - hx * dz, tz = hx * dy - hy * dx;
+    const TS _t = sqrt(tx * tx + ty * ty + tz * tz);
+    acc[0] += D(A1 * g * dx / r + A2 * g * tx / _t + A3 * g * hx / h);
+    acc[1] += D(A1 * g * dy / r + A2 * g * ty / _t + A3 * g * hy / h);
+    acc[2] += D(A1 * g * dz / r + A2 * g * tz / _t + A3 * g * hz / h);
+    if (!need_jac) return;
+    const TS r3 = r * r * r;
+    const TS v2 = dvx * dvx + dvy * dvy + dvz * dvz;
+    const TS rdotv = dx * dvx + dy * dvy + dz * dvz;
+    const TS vdott = dvx * tx + dvy * ty + dvz * tz;
+    const TS dgdr = (alpha / r0) * (-nm * pow(r / r0, -nm - TS(1)) * pow(TS(1.0) + pow(r / r0, nn), -nk) +
+                                    pow(r / r0, -nm) * (-nk * nn) * pow(r / r0, nn - TS(1)) * pow(TS(1.0) + pow(r / r0, nn), -nk - TS(1)));
+    const TS dgx = dgdr * dx / r, dgy = dgdr * dy / r, dgz = dgdr * dz / r;
+    const TS h3 = h * h * h, t3 = _t * _t * _t;
+    const TS hxh3 = hx / h3, hyh3 = hy / h3, hzh3 = hz / h3;
+    const TS txt3 = tx / t3, tyt3 = ty / t3, tzt3 = tz / t3;
+    J.A[0][0] += g * dx / r; J.A[1][0] += g * dy / r; J.A[2][0] += g * dz / r;
+    J.A[0][1] += g * tx / _t; J.A[1][1] += g * ty / _t; J.A[2][1] += g * tz / _t;
+    J.A[0][2] += g * hx / h; J.A[1][2] += g * hy / h; J.A[2][2] += g * hz / h;
+    const TS one = TS(1), two = TS(2);
+    J.x[0][0] += A1 * (dgx * dx / r + g * (one / r - dx * dx / r3)) + A2 * (dgx * tx / _t + g * ((dx * dvx - rdotv) / _t - txt3 * (two * dx * vdott - rdotv * tx))) + A3 * (dgx * hx / h + g * (-hxh3) * (v2 * dx - rdotv * dvx));
+    J.x[1][1] += A1 * (dgy * dy / r + g * (one / r - dy * dy / r3)) + A2 * (dgy * ty / _t + g * ((dy * dvy - rdotv) / _t - tyt3 * (two * dy * vdott - rdotv * ty))) + A3 * (dgy * hy / h + g * (-hyh3) * (v2 * dy - rdotv * dvy));
+    J.x[2][2] += A1 * (dgz * dz / r + g * (one / r - dz * dz / r3)) + A2 * (dgz * tz / _t + g * ((dz * dvz - rdotv) / _t - tzt3 * (two * dz * vdott - rdotv * tz))) + A3 * (dgz * hz / h + g * (-hzh3) * (v2 * dz - rdotv * dvz));
+    J.x[0][1] += A1 * (dgy * dx / r + g * (-dx * dy / r3)) + A2 * (dgy * tx / _t + g * ((two * dy * dvx - dx * dvy) / _t - txt3 * (two * dy * vdott - rdotv * ty))) + A3 * (dgy * hx / h + g * (dvz / h - hxh3 * (v2 * dy - rdotv * dvy)));
+    J.x[1][0] += A1 * (dgx * dy / r + g * (-dy * dx / r3)) + A2 * (dgx * ty / _t + g * ((two * dx * dvy - dy * dvx) / _t - tyt3 * (two * dx * vdott - rdotv * tx))) + A3 * (dgx * hy / h + g * (-dvz / h - hyh3 * (v2 * dx - rdotv * dvx)));
+    J.x[0][2] += A1 * (dgz * dx / r + g * (-dx * dz / r3)) + A2 * (dgz * tx / _t + g * ((two * dz * dvx - dx * dvz) / _t - txt3 * (two * dz * vdott - rdotv * tz))) + A3 * (dgz * hx / h + g * (-dvy / h - hxh3 * (v2 * dz - rdotv * dvz)));
+    J.x[2][0] += A1 * (dgx * dz / r + g * (-dz * dx / r3)) + A2 * (dgx * tz / _t + g * ((two * dx * dvz - dz * dvx) / _t - tzt3 * (two * dx * vdott - rdotv * tx))) + A3 * (dgx * hz / h + g * (dvy / h - hzh3 * (v2 * dx - rdotv * dvx)));
+    J.x[1][2] += A1 * (dgz * dy / r + g * (-dy * dz / r3)) + A2 * (dgz * ty / _t + g * ((two * dz * dvy - dy * dvz) / _t - tyt3 * (two * dz * vdott - rdotv * tz))) + A3 * (dgz * hy / h + g * (dvx / h - hyh3 * (v2 * dz - rdotv * dvz)));
+    J.x[2][1] += A1 * (dgy * dz / r + g * (-dz * dy / r3)) + A2 * (dgy * tz / _t + g * ((two * dy * dvz - dz * dvy) / _t - tzt3 * (two * dy * vdott - rdotv * ty))) + A3 * (dgy * hz / h + g * (-dvx / h - hzh3 * (v2 * dy - rdotv * dvy)));
+    J.v[0][0] += A2 * g * ((dy * dy + dz * dz) / _t - txt3 * r2 * tx) + A3 * g * (-hxh3 * (r2 * dvx - dx * rdotv));
+    J.v[1][1] += A2 * g * ((dx * dx + dz * dz) / _t - tyt3 * r2 * ty) + A3 * g * (-hyh3 * (r2 * dvy - dy * rdotv));
+    J.v[2][2] += A2 * g * ((dx * dx + dy * dy) / _t - tzt3 * r2 * tz) + A3 * g * (-hzh3 * (r2 * dvz - dz * rdotv));
+    J.v[0][1] += A2 * g * (-dy * dx / _t - tyt3 * r2 * tx) + A3 * g * (-dz / h - hxh3 * (r2 * dvy - dy * rdotv));
+    J.v[1][0] += A2 * g * (-dx * dy / _t - txt3 * r2 * ty) + A3 * g * (dz / h - hyh3 * (r2 * dvx - dx * rdotv));
+    J.v[0][2] += A2 * g * (-dz * dx / _t - tzt3 * r2 * tx) + A3 * g * (dy / h - hxh3 * (r2 * dvz - dz * rdotv));
+    J.v[2][0] += A2 * g * (-dx * dz / _t - txt3 * r2 * tz) + A3 * g * (-dy / h - hzh3 * (r2 * dvx - dx * rdotv));
+    J.v[1][2] += A2 * g * (-dz * dy / _t - tzt3 * r2 * ty) + A3 * g * (-dx / h - hyh3 * (r2 * dvz - dz * rdotv));
+    J.v[2][1] += A2 * g * (-dy * dz / _t - tyt3 * r2 * tz) + A3 * g * (dx / h - hzh3 * (r2 * dvy - dy * rdotv));
+}
+
+// EIH GR with the Sun as the only source (reference src/forces.c:1288-1983, beta = gamma = 1),
+// given the group-summed planet sums: S0 = sum_k GM_k / r_ik, dS0 = -sum_k GM_k d_ik / r_ik^3
+// (k over the 11 planetary bodies), S1 = sum_{k != Sun} GM_k / r_sk and the Sun's Newtonian
+// acceleration aj from the same bodies.  In TS.
+template <typename D, typename TS>
+AHD void term_eih_sun(const DevEphem& E, const D xr[3], const D vr[3], const D xs[3], const D vs[3], D GMs, D S0,
+                      const D dS0[3], D S1, const D aj[3], bool need_jac, D acc[3], Jac<TS>& J) {
+    const TS oc2 = TS(E.over_c_squared);
+    const TS dxij = TS(xr[0] - xs[0]), dyij = TS(xr[1] - xs[1]), dzij = TS(xr[2] - xs[2]);
+    const TS vx = TS(vr[0]), vy = TS(vr[1]), vz = TS(vr[2]);
+    const TS vxj = TS(vs[0]), vyj = TS(vs[1]), vzj = TS(vs[2]);
+    const TS GMj = TS(GMs);
+    const TS rij2 = dxij * dxij + dyij * dyij + dzij * dzij;
+    const TS _rij = sqrt(rij2);
+    const TS prefacij = GMj / (rij2 * _rij);
+    const TS vi2 = vx * vx + vy * vy + vz * vz;
+    const TS term2 = oc2 * vi2;
+    const TS vj2 = vxj * vxj + vyj * vyj + vzj * vzj;
+    const TS term3 = TS(2) * oc2 * vj2;
+    const TS vidotvj = vx * vxj + vy * vyj + vz * vzj;
+    const TS term4 = TS(-4) * oc2 * vidotvj;
+    const TS rijdotvj = dxij * vxj + dyij * vyj + dzij * vzj;
+    const TS term5 = TS(-1.5) * oc2 * (rijdotvj * rijdotvj) / (_rij * _rij);
+    const TS fx = TS(4) * vx - TS(3) * vxj, fy = TS(4) * vy - TS(3) * vyj, fz = TS(4) * vz - TS(3) * vzj;
+    const TS f = dxij * fx + dyij * fy + dzij * fz;
+    const TS pf = prefacij * f;
+    const TS rel[3] = {vx - vxj, vy - vyj, vz - vzj};
+    const TS term0 = TS(S0) * (TS(-4) * oc2);
+    const TS term1 = TS(S1) * (-oc2);
+    const TS axj = TS(aj[0]), ayj = TS(aj[1]), azj = TS(aj[2]);
+    const TS term6 = TS(-0.5) * oc2 * (dxij * axj + dyij * ayj + dzij * azj);
+    const TS t8 = GMj / _rij * TS(3.5);
+    const TS factor = term0 + term1 + term2 + term3 + term4 + term5 + term6;
+    const TS dij[3] = {dxij, dyij, dzij};
+    const TS ajv[3] = {axj, ayj, azj};
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[c] += D(-prefacij * dij[c] * factor + (pf * rel[c] * oc2 + t8 * ajv[c] * oc2));
+    if (!need_jac) return;
+    const TS pv = GMj / (_rij * _rij * _rij);
+    const TS r5 = _rij * _rij * _rij * _rij * _rij;
+    const TS dpref[3] = {TS(-3.0) * GMj / r5 * dxij, TS(-3.0) * GMj / r5 * dyij, TS(-3.0) * GMj / r5 * dzij};
+    const TS t5f = TS(3.0) * oc2 * rijdotvj / _rij;
+    const TS r3 = _rij * _rij * _rij;
+    const TS vj[3] = {vxj, vyj, vzj};
+    const TS vi[3] = {vx, vy, vz};
+    const TS fv[3] = {fx, fy, fz};
+    TS dfx[3], dfv[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const TS dt5 = -t5f * (vj[q] / _rij - rijdotvj * dij[q] / r3);
+        const TS dt6 = TS(-0.5) * oc2 * ajv[q];
+        dfx[q] = TS(dS0[q]) * (TS(-4) * oc2) + dt5 + dt6;
+        dfv[q] = TS(2.0) * oc2 * vi[q] + TS(-4) * oc2 * vj[q];
+    }
+    const TS pf7 = pv * f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const TS d7x = dpref[q] * f * rel[c] + pv * fv[q] * rel[c];
+            TS d7v = pv * (TS(4) * dij[q]) * rel[c];
+            if (q == c) d7v += pf7;
+            const TS d8 = -GMj * ajv[c] / r3 * dij[q] * TS(3.5);
+            TS v = -dpref[q] * dij[c] * factor - pv * dij[c] * dfx[q];
+            if (q == c) v -= pv * factor;
+            J.x[c][q] += v + (d7x * oc2 + d8 * oc2);
+            J.v[c][q] += -pv * dij[c] * dfv[q] + d7v * oc2;
+        }
+    }
+}
+
+// GR potential form (reference src/forces.c:1059-1161), Sun only, in TS.
+template <typename D, typename TS>
+AHD void term_gr_potential(const DevEphem& E, const D xr[3], const D xs[3], D GMs, bool need_jac, D acc[3], Jac<TS>& J) {
+    const TS px = TS(xr[0] - xs[0]), py = TS(xr[1] - xs[1]), pz = TS(xr[2] - xs[2]);
+    const TS r2 = px * px + py * py + pz * pz;
+    const TS r = sqrt(r2);
+    const TS G = TS(GMs);
+    const TS prefac = TS(-6.0) * G * G / (TS(E.c_squared) * r2 * r2);
+    acc[0] += D(prefac * px);
+    acc[1] += D(prefac * py);
+    acc[2] += D(prefac * pz);
+    if (!need_jac) return;
+    const TS u[3] = {px / r, py / r, pz / r};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) J.x[i][j] += (i == j ? prefac : TS(0)) + TS(-4.0) * prefac * u[i] * u[j];
+}
+
+// GR simple form (reference src/forces.c:1163-1286), Sun only, in TS.
+template <typename D, typename TS>
+AHD void term_gr_simple(const DevEphem& E, const D xr[3], const D vr[3], const D xs[3], const D vs[3], D GMs,
+                        bool need_jac, D acc[3], Jac<TS>& J) {
+    const TS p[3] = {TS(xr[0] - xs[0]), TS(xr[1] - xs[1]), TS(xr[2] - xs[2])};
+    const TS pv[3] = {TS(vr[0] - vs[0]), TS(vr[1] - vs[1]), TS(vr[2] - vs[2])};
+    const TS G = TS(GMs);
+    const TS v2 = pv[0] * pv[0] + pv[1] * pv[1] + pv[2] * pv[2];
+    const TS r = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    const TS A = TS(4.0) * G / r - v2;
+    const TS B = TS(4.0) * (p[0] * pv[0] + p[1] * pv[1] + p[2] * pv[2]);
+    const TS prefac = G / (r * r * r * TS(E.c_squared));
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[c] += D(prefac * (A * p[c] + B * pv[c]));
+    if (!need_jac) return;
+    const TS dpdr = TS(-3.0) * prefac / r;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const TS Fi = A * p[i] + B * pv[i];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            J.x[i][j] += dpdr * p[j] / r * Fi + prefac * ((i == j ? A : TS(0)) - p[i] * (p[j] / r) * TS(4.0) * G / (r * r) + TS(4.0) * pv[j] * pv[i]);
+            J.v[i][j] += prefac * (TS(-2.0) * pv[j] * p[i] + TS(4.0) * p[j] * pv[i] + (i == j ? B : TS(0)));
+        }
+    }
+}
+
+// ---- the full evaluation ---------------------------------------------------------------------
+template <typename D>
+struct EvalResult {
+    D acc[3];    // real-particle acceleration
+    int status;
+};
+
+template <typename D, bool Mixed>
+struct TermScalar { using type = typename std::conditional<Mixed, typename SingleOf<D>::type, D>::type; };
+
+// Evaluates the force model for the real particle at (t, xr, vr) cooperatively over the G lanes
+// of a group; every lane receives the real acceleration and the Jacobian J.
+template <int G, bool Mixed, typename D>
+AHD void force_eval(const DevEphem& E, const ForceOpts& O, int lane, D t, const D xr[3], const D vr[3],
+                    const double A[3], bool ng, bool need_jac, EvalResult<D>& out,
+                    Jac<typename TermScalar<D, Mixed>::type>& J) {
+    using TS = typename TermScalar<D, Mixed>::type;
+    constexpr int NR = (kNumSlots + G - 1) / G;
+    const int forces = O.forces;
+    const double tv = value_of(t);
+    out.status = (tv >= E.t_beg && tv <= E.t_end) ? kOk : kCoverage;
+    const bool eih = (forces & F_GR_EIH) && O.gr_eih_sources > 0;
+    const bool want_sun_vel = eih || ((forces & F_NONGRAV) && ng) || (forces & F_GR_SIMPLE);
+
+    // 1. Chebyshev evaluations of the slots owned by this lane
+    D px[NR][3];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int s = r * G + lane;
+        if (s < kNumSlots) {
+            const ChebOut<D> c = cheb_eval<D>(E.slot[s], t, false);
+            px[r][0] = c.x; px[r][1] = c.y; px[r][2] = c.z;
+        } else {
+            px[r][0] = px[r][1] = px[r][2] = D(0.0);
+        }
+    }
+    D sunv_l[3] = {D(0.0), D(0.0), D(0.0)};
+    if (want_sun_vel && lane == 0) {
+        const ChebOut<D> c = cheb_eval<D>(E.slot[0], t, true);
+        sunv_l[0] = c.vx; sunv_l[1] = c.vy; sunv_l[2] = c.vz;
+    }
+    // 2. exchange Sun (au) and EMB (km); compose Earth, Moon and asteroids
+    D sun[3], sunv[3], emb[3];
+    const D inv_au = D(E.inv_au);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        sun[c] = gbcast<G>(px[0][c] * inv_au, 0);
+        sunv[c] = gbcast<G>(sunv_l[c] * inv_au, 0);
+        emb[c] = gbcast<G>(px[kSlotEMB / G][c], kSlotEMB % G);
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int s = r * G + lane;
+        if (s >= kNumSlots) continue;
+        if (s < kFirstAsteroidSlot) {
+            const bool em = (s == kSlotEarth || s == kSlotMoon);
+#pragma unroll
+            for (int c = 0; c < 3; c++) px[r][c] = (em ? px[r][c] + emb[c] : px[r][c]) * inv_au;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; c++) px[r][c] = fma(px[r][c], D(E.inv_au_ast), sun[c]);
+        }
+    }
+    D earth[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) earth[c] = gbcast<G>(px[kSlotEarth / G][c], kSlotEarth % G);
+
+    // 3. per-body point masses, Jacobian and EIH planet sums on the owning lane
+    D ad[3] = {D(0.0), D(0.0), D(0.0)};  // direct acceleration, FP64 bodies
+    TS adt[3] = {TS(0), TS(0), TS(0)};   // direct acceleration, TS bodies (asteroids in mixed mode)
+    TS jd[6] = {TS(0), TS(0), TS(0), TS(0), TS(0), TS(0)};  // xx yy zz xy yz xz
+    D S0 = D(0.0), S1 = D(0.0), dS0[3] = {D(0.0), D(0.0), D(0.0)}, aj[3] = {D(0.0), D(0.0), D(0.0)};
+    const int nbody = kNumPlanets + E.n_ast;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int s = r * G + lane;
+        if (s >= kNumSlots || s == kSlotEMB) continue;
+        const int b = slot_body(s);
+        if (b >= nbody) continue;
+        const D GM = D(E.GM[b]);
+        const bool allowed = (b == 0) ? (forces & F_SUN) : (b < kNumPlanets ? (forces & F_PLANETS) : (forces & F_ASTEROIDS));
+        const D dx = xr[0] - px[r][0], dy = xr[1] - px[r][1], dz = xr[2] - px[r][2];
+        if (Mixed && b >= kNumPlanets) {
+            const TS fx = TS(dx), fy = TS(dy), fz = TS(dz);
+            const TS r2 = fx * fx + fy * fy + fz * fz;
+            const TS rr = sqrt(r2);
+            const TS r3inv = TS(1) / (r2 * rr);
+            const TS pref = TS(GM) * r3inv;
+            if (allowed) { adt[0] -= pref * fx; adt[1] -= pref * fy; adt[2] -= pref * fz; }
+            if (need_jac) {
+                const TS r5 = TS(3) * r3inv / r2;
+                const TS g = TS(GM);
+                jd[0] += g * (fx * fx * r5 - r3inv); jd[1] += g * (fy * fy * r5 - r3inv); jd[2] += g * (fz * fz * r5 - r3inv);
+                jd[3] += g * (fx * fy * r5); jd[4] += g * (fy * fz * r5); jd[5] += g * (fx * fz * r5);
+            }
+            continue;
+        }
+        const D r2 = dx * dx + dy * dy + dz * dz;
+        const D rr = sqrt(r2);
+        const D r3inv = D(1.0) / (r2 * rr);
+        const D pref = GM * r3inv;
+        const D ax = -pref * dx, ay = -pref * dy, az = -pref * dz;
+        if (allowed) { ad[0] += ax; ad[1] += ay; ad[2] += az; }
+        if (need_jac) {
+            const TS r3t = TS(r3inv);
+            const TS r5 = TS(D(3.0) * r3inv / r2);
+            const TS g = TS(GM), fx = TS(dx), fy = TS(dy), fz = TS(dz);
+            jd[0] += g * (fx * fx * r5 - r3t); jd[1] += g * (fy * fy * r5 - r3t); jd[2] += g * (fz * fz * r5 - r3t);
+            jd[3] += g * (fx * fy * r5); jd[4] += g * (fy * fz * r5); jd[5] += g * (fx * fz * r5);
+        }
+        if (eih && b < kNumPlanets) {
+            S0 += GM / rr;
+            dS0[0] += ax; dS0[1] += ay; dS0[2] += az;
+            if (b != 0) {
+                const D sx = sun[0] - px[r][0], sy = sun[1] - px[r][1], sz = sun[2] - px[r][2];
+                const D q2 = sx * sx + sy * sy + sz * sz;
+                const D q = sqrt(q2);
+                S1 += GM / q;
+                const D f = GM / (q2 * q);
+                aj[0] -= f * sx; aj[1] -= f * sy; aj[2] -= f * sz;
+            }
+        }
+    }
+    // 4. group sums
+    if constexpr (G > 1) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) { ad[c] = gsum<G>(ad[c]); adt[c] = gsum<G>(adt[c]); }
+        if (need_jac) {
+#pragma unroll
+            for (int k = 0; k < 6; k++) jd[k] = gsum<G>(jd[k]);
+        }
+        if (eih) {
+            S0 = gsum<G>(S0); S1 = gsum<G>(S1);
+#pragma unroll
+            for (int c = 0; c < 3; c++) { dS0[c] = gsum<G>(dS0[c]); aj[c] = gsum<G>(aj[c]); }
+        }
+    }
+    // 5. single-particle terms, reference order
+    D acc[3] = {D(0.0), D(0.0), D(0.0)};
+    jac_zero(J);
+    const D GMs = D(E.GM[0]);
+    if ((forces & F_NONGRAV) && ng) term_nongrav<D, TS>(O, xr, vr, sun, sunv, A, need_jac, acc, J);
+    if (forces & F_EARTH_H) term_earth_J2J4<D, TS>(E, xr, earth, D(E.GM[3]), need_jac, acc, J);
+    if (forces & F_SUN_H) term_solar_J2<D, TS>(E, xr, sun, GMs, need_jac, acc, J);
+    if (eih) term_eih_sun<D, TS>(E, xr, vr, sun, sunv, GMs, S0, dS0, S1, aj, need_jac, acc, J);
+    if (forces & F_GR_POTENTIAL) term_gr_potential<D, TS>(E, xr, sun, GMs, need_jac, acc, J);
+    if (forces & F_GR_SIMPLE) term_gr_simple<D, TS>(E, xr, vr, sun, sunv, GMs, need_jac, acc, J);
+    if (forces & (F_SUN | F_PLANETS | F_ASTEROIDS)) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[c] += ad[c] + D(adt[c]);
+        if (need_jac) {
+            J.x[0][0] += jd[0]; J.x[1][1] += jd[1]; J.x[2][2] += jd[2];
+            J.x[0][1] += jd[3]; J.x[1][0] += jd[3];
+            J.x[1][2] += jd[4]; J.x[2][1] += jd[4];
+            J.x[0][2] += jd[5]; J.x[2][0] += jd[5];
+        }
+    }
+    out.acc[0] = acc[0]; out.acc[1] = acc[1]; out.acc[2] = acc[2];
+}
+
+// Acceleration of this lane's own vector: the real particle (lane 0) or a variational
+// particle (J applied to its state and its Marsden-parameter variation).
+template <typename D, typename TS>
+AHD void own_accel(int lane, const EvalResult<D>& R, const Jac<TS>& J, const D x[3], const D v[3], const double dA[3],
+                   bool ng, D out[3]) {
+    if (lane == 0) {
+        out[0] = R.acc[0]; out[1] = R.acc[1]; out[2] = R.acc[2];
+        return;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        TS s = J.x[c][0] * TS(x[0]) + J.x[c][1] * TS(x[1]) + J.x[c][2] * TS(x[2]);
+        s += J.v[c][0] * TS(v[0]) + J.v[c][1] * TS(v[1]) + J.v[c][2] * TS(v[2]);
+        if (ng) s += J.A[c][0] * TS(dA[0]) + J.A[c][1] * TS(dA[1]) + J.A[c][2] * TS(dA[2]);
+        out[c] = D(s);
+    }
+}
+
+}  // namespace assist_gpu

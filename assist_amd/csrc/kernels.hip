@@ -1,0 +1,514 @@
+// Batched ASSIST kernels for gfx950.
+//
+//  * batch_forces_kernel: one force evaluation per system (parity and the
+//    assist_additional_forces hook).
+//  * ias15_kernel: a persistent integrator.  Each group of G lanes owns one
+//    system at a time; lane k of the group holds the IAS15 state of vector k
+//    (k = 0 the real particle, k >= 1 its variational particles) in registers.
+//    Every trip round the loop performs exactly one force evaluation for every
+//    active group, whatever phase of its step the group is in (start-of-step
+//    acceleration, predictor-corrector substep n, after a rejection), so the
+//    expensive part runs with the whole wave converged.  Finished groups take
+//    the next system from an atomic work queue.
+//
+// IAS15 follows Rein & Spiegel (2015), MNRAS 446, 1424, as configured by ASSIST
+// (global step-size criterion, reference src/assist.c:440-446).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "device_types.h"
+#include "force_model.h"
+#include "ias15_tables.h"
+#include "kernels.h"
+
+namespace assist_gpu {
+
+__constant__ double cH[8] = IAS15_H_INIT;
+__constant__ double cRRI[28] = IAS15_RRINV_INIT;
+__constant__ double cC[21] = IAS15_C_INIT;
+__constant__ double cD[21] = IAS15_D_INIT;
+
+// ------------------------------------------------------------------------------------------
+// batched force evaluation
+// ------------------------------------------------------------------------------------------
+template <int G, bool Mixed>
+__global__ void __launch_bounds__(256) batch_forces_kernel(ForcesArgs a) {
+    using TS = typename TermScalar<double, Mixed>::type;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & (G - 1);
+    const int sys = tid / G;
+    if (sys >= a.nsys) return;  // whole groups exit together (nsys*G threads, G | blockDim)
+    const int nv = a.nvec;
+    const double* st = a.states + (size_t)sys * nv * 6;
+    double xr[3], vr[3], A[3] = {0.0, 0.0, 0.0}, dA[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 3; c++) { xr[c] = st[c]; vr[c] = st[3 + c]; }
+    bool ng = false;
+    if (a.params) {
+        const double* pp = a.params + (size_t)sys * nv * 3;
+        A[0] = pp[0]; A[1] = pp[1]; A[2] = pp[2];
+        ng = !(A[0] == 0.0 && A[1] == 0.0 && A[2] == 0.0);
+        if (lane < nv) { dA[0] = pp[3 * lane]; dA[1] = pp[3 * lane + 1]; dA[2] = pp[3 * lane + 2]; }
+    }
+    EvalResult<double> R;
+    Jac<TS> J;
+    force_eval<G, Mixed, double>(*a.E, a.fo, lane, a.t[sys], xr, vr, A, ng, nv > 1, R, J);
+    // each lane handles vectors lane, lane+G, ... (G may be smaller than nvec for parity runs)
+    for (int v = lane; v < nv; v += G) {
+        double x[3], vv[3], out[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) { x[c] = st[6 * v + c]; vv[c] = st[6 * v + 3 + c]; }
+        double dAv[3] = {0.0, 0.0, 0.0};
+        if (a.params) {
+            const double* pp = a.params + ((size_t)sys * nv + v) * 3;
+            dAv[0] = pp[0]; dAv[1] = pp[1]; dAv[2] = pp[2];
+        }
+        own_accel<double, TS>(v == 0 ? 0 : 1, R, J, x, vv, dAv, ng, out);
+#pragma unroll
+        for (int c = 0; c < 3; c++) a.acc[((size_t)sys * nv + v) * 3 + c] = out[c];
+    }
+    if (lane == 0 && a.status) a.status[sys] = R.status;
+}
+
+// ------------------------------------------------------------------------------------------
+// IAS15
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void add_cs(double& p, double& cs, double in) {
+    const double y = in - cs;
+    const double t = p + y;
+    cs = (t - p) - y;
+    p = t;
+}
+
+// Re-expansion of the last accepted step's b (br) for a step of ratio q:
+// e = Q(q) br (Taylor re-centring at h = 1, scaled by q^k); b = e + (br - er).
+__device__ __forceinline__ void predict_e(const double br[7], double q, double e[7]) {
+    const double q2 = q * q, q3 = q * q2, q4 = q2 * q2, q5 = q2 * q3, q6 = q3 * q3, q7 = q3 * q4;
+    e[0] = q * (br[6] * 7.0 + br[5] * 6.0 + br[4] * 5.0 + br[3] * 4.0 + br[2] * 3.0 + br[1] * 2.0 + br[0]);
+    e[1] = q2 * (br[6] * 21.0 + br[5] * 15.0 + br[4] * 10.0 + br[3] * 6.0 + br[2] * 3.0 + br[1]);
+    e[2] = q3 * (br[6] * 35.0 + br[5] * 20.0 + br[4] * 10.0 + br[3] * 4.0 + br[2]);
+    e[3] = q4 * (br[6] * 35.0 + br[5] * 15.0 + br[4] * 5.0 + br[3]);
+    e[4] = q5 * (br[6] * 21.0 + br[5] * 6.0 + br[4]);
+    e[5] = q6 * (br[6] * 7.0 + br[5]);
+    e[6] = q7 * br[6];
+}
+
+template <int G, bool Mixed>
+__global__ void __launch_bounds__(256) ias15_kernel(IntegrateArgs a) {
+    using TS = typename TermScalar<double, Mixed>::type;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & (G - 1);
+    const int grp = tid / G;
+    const int nv = a.nvec;
+    const bool own = lane < nv;
+    const DevEphem& E = *a.E;
+    const IntegratorOpts io = a.io;
+    // br[7][3], er[7][3] of the last accepted step, per lane, in the scratch buffer
+    double* scr = a.scratch + ((size_t)grp * G + lane) * 42;
+
+    double x0[3], v0[3], csx[3], csv[3], a0[3], b[7][3], g[7][3];
+    double A[3], dA[3];
+    double t = 0, t_end = 0, dt = 0, dts = 0, dt_last = 0, pc_err_last = 0;
+    int phase = 0, n = 1, it = 0, last = 0, status = 0, attempts = 0;
+    long long steps = 0, evals = 0, rejected = 0;
+    bool ng = false;
+    int sys = grp;
+    bool active = sys < a.nsys;
+
+    auto load = [&](int s) {
+        const double* st = a.states + ((size_t)s * nv + (own ? lane : 0)) * 6;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            x0[c] = own ? st[c] : 0.0;
+            v0[c] = own ? st[3 + c] : 0.0;
+            csx[c] = csv[c] = a0[c] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 7; k++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) b[k][c] = g[k][c] = 0.0;
+        A[0] = A[1] = A[2] = dA[0] = dA[1] = dA[2] = 0.0;
+        ng = false;
+        if (a.params) {
+            const double* pp = a.params + (size_t)s * nv * 3;
+            A[0] = pp[0]; A[1] = pp[1]; A[2] = pp[2];
+            ng = !(A[0] == 0.0 && A[1] == 0.0 && A[2] == 0.0);
+            if (own) { dA[0] = pp[3 * lane]; dA[1] = pp[3 * lane + 1]; dA[2] = pp[3 * lane + 2]; }
+        }
+        t = a.t0[s];
+        t_end = a.t1[s];
+        dt = io.dt_initial;
+        if (dt * (t_end - t) < 0) dt = -dt;
+        dt_last = 0.0;
+        phase = 0;
+        status = 0;
+        steps = evals = rejected = 0;
+        last = 0;
+    };
+    auto finish = [&]() {
+        if (own) {
+            double* st = a.states + ((size_t)sys * nv + lane) * 6;
+#pragma unroll
+            for (int c = 0; c < 3; c++) { st[c] = x0[c]; st[3 + c] = v0[c]; }
+        }
+        if (lane == 0) {
+            if (a.status) a.status[sys] = status;
+            if (a.steps) a.steps[sys] = steps;
+            if (a.evals) a.evals[sys] = evals;
+            if (a.rejected) a.rejected[sys] = rejected;
+            if (a.t_out) a.t_out[sys] = t;
+        }
+        int next = 0;
+        if (lane == 0) next = atomicAdd(a.queue, 1);
+        next = gbcast_i<G>(next, 0);
+        sys = next;
+        active = sys < a.nsys;
+        if (active) load(sys);
+    };
+    // Prepare an attempt of the current step: trim to t_end, predict b from the last
+    // accepted step, convert to g.  Returns false when the step cannot be taken.
+    auto setup_attempt = [&](bool new_step) -> bool {
+        const double dir = (t_end >= t) ? 1.0 : -1.0;
+        if (new_step) {
+            dts = dt;
+            last = 0;
+            if ((t + dts - t_end) * dir >= 0) { dts = t_end - t; last = 1; }
+            attempts = 0;
+        }
+        if (!(fabs(dts) > 0.0) || !(fabs(dts) < 1e300) || ++attempts > 64) { status = kDtUnderflow; return false; }
+        if (dt_last != 0.0 && own) {
+            const double q = dts / dt_last;
+            if (q > 20.0) {
+#pragma unroll
+                for (int k = 0; k < 7; k++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) b[k][c] = 0.0;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    double br[7], er[7], e[7];
+#pragma unroll
+                    for (int k = 0; k < 7; k++) { br[k] = scr[3 * k + c]; er[k] = scr[21 + 3 * k + c]; }
+                    predict_e(br, q, e);
+#pragma unroll
+                    for (int k = 0; k < 7; k++) b[k][c] = e[k] + (br[k] - er[k]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 7; k++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) b[k][c] = 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double b0 = b[0][c], b1 = b[1][c], b2 = b[2][c], b3 = b[3][c], b4 = b[4][c], b5 = b[5][c], b6 = b[6][c];
+            g[0][c] = b6 * cD[15] + b5 * cD[10] + b4 * cD[6] + b3 * cD[3] + b2 * cD[1] + b1 * cD[0] + b0;
+            g[1][c] = b6 * cD[16] + b5 * cD[11] + b4 * cD[7] + b3 * cD[4] + b2 * cD[2] + b1;
+            g[2][c] = b6 * cD[17] + b5 * cD[12] + b4 * cD[8] + b3 * cD[5] + b2;
+            g[3][c] = b6 * cD[18] + b5 * cD[13] + b4 * cD[9] + b3;
+            g[4][c] = b6 * cD[19] + b5 * cD[14] + b4;
+            g[5][c] = b6 * cD[20] + b5;
+            g[6][c] = b6;
+        }
+        it = 1;
+        pc_err_last = 1e300;
+        n = 1;
+        phase = 1;
+        return true;
+    };
+
+    if (active) load(sys);
+    while (active) {
+        if (phase == 0 && t == t_end) { finish(); continue; }  // zero-length span
+        // ---- evaluation point of this trip ----
+        double xe[3], ve[3], te;
+        if (phase == 0) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) { xe[c] = x0[c]; ve[c] = v0[c]; }
+            te = t;
+        } else {
+            const double h = cH[n];
+            double s[9];
+            s[0] = dts * h;
+            s[1] = s[0] * s[0] / 2.;
+            s[2] = s[1] * h / 3.;
+            s[3] = s[2] * h / 2.;
+            s[4] = 3. * s[3] * h / 5.;
+            s[5] = 2. * s[4] * h / 3.;
+            s[6] = 5. * s[5] * h / 7.;
+            s[7] = 3. * s[6] * h / 4.;
+            s[8] = 7. * s[7] * h / 9.;
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                xe[c] = -csx[c] + ((s[8] * b[6][c] + s[7] * b[5][c] + s[6] * b[4][c] + s[5] * b[3][c] + s[4] * b[2][c] +
+                                    s[3] * b[1][c] + s[2] * b[0][c] + s[1] * a0[c] + s[0] * v0[c]) + x0[c]);
+            s[1] = s[0] * h / 2.;
+            s[2] = 2. * s[1] * h / 3.;
+            s[3] = 3. * s[2] * h / 4.;
+            s[4] = 4. * s[3] * h / 5.;
+            s[5] = 5. * s[4] * h / 6.;
+            s[6] = 6. * s[5] * h / 7.;
+            s[7] = 7. * s[6] * h / 8.;
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                ve[c] = -csv[c] + ((s[7] * b[6][c] + s[6] * b[5][c] + s[5] * b[4][c] + s[4] * b[3][c] + s[3] * b[2][c] +
+                                    s[2] * b[1][c] + s[1] * b[0][c] + s[0] * a0[c]) + v0[c]);
+            te = t + s[0];
+        }
+        // ---- force evaluation (whole group) ----
+        double xr[3], vr[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) { xr[c] = gbcast<G>(xe[c], 0); vr[c] = gbcast<G>(ve[c], 0); }
+        EvalResult<double> R;
+        Jac<TS> J;
+        force_eval<G, Mixed, double>(E, a.fo, lane, te, xr, vr, A, ng, nv > 1, R, J);
+        double at[3];
+        own_accel<double, TS>(lane, R, J, xe, ve, dA, ng, at);
+        evals++;
+        if (R.status != kOk) { status = R.status; finish(); continue; }
+
+        if (phase == 0) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) a0[c] = at[c];
+            if (!setup_attempt(true)) { finish(); continue; }
+            continue;
+        }
+        // ---- corrector: improve g_n and b ----
+        double maxak = 0.0, maxdb6 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double gk = at[c] - a0[c];
+            double tmp;
+            switch (n) {
+            case 1:
+                tmp = g[0][c];
+                g[0][c] = gk * cRRI[0];
+                b[0][c] += g[0][c] - tmp;
+                break;
+            case 2:
+                tmp = g[1][c];
+                g[1][c] = (gk * cRRI[1] - g[0][c]) * cRRI[2];
+                tmp = g[1][c] - tmp;
+                b[0][c] += tmp * cC[0];
+                b[1][c] += tmp;
+                break;
+            case 3:
+                tmp = g[2][c];
+                g[2][c] = ((gk * cRRI[3] - g[0][c]) * cRRI[4] - g[1][c]) * cRRI[5];
+                tmp = g[2][c] - tmp;
+                b[0][c] += tmp * cC[1];
+                b[1][c] += tmp * cC[2];
+                b[2][c] += tmp;
+                break;
+            case 4:
+                tmp = g[3][c];
+                g[3][c] = (((gk * cRRI[6] - g[0][c]) * cRRI[7] - g[1][c]) * cRRI[8] - g[2][c]) * cRRI[9];
+                tmp = g[3][c] - tmp;
+                b[0][c] += tmp * cC[3];
+                b[1][c] += tmp * cC[4];
+                b[2][c] += tmp * cC[5];
+                b[3][c] += tmp;
+                break;
+            case 5:
+                tmp = g[4][c];
+                g[4][c] = ((((gk * cRRI[10] - g[0][c]) * cRRI[11] - g[1][c]) * cRRI[12] - g[2][c]) * cRRI[13] - g[3][c]) * cRRI[14];
+                tmp = g[4][c] - tmp;
+                b[0][c] += tmp * cC[6];
+                b[1][c] += tmp * cC[7];
+                b[2][c] += tmp * cC[8];
+                b[3][c] += tmp * cC[9];
+                b[4][c] += tmp;
+                break;
+            case 6:
+                tmp = g[5][c];
+                g[5][c] = (((((gk * cRRI[15] - g[0][c]) * cRRI[16] - g[1][c]) * cRRI[17] - g[2][c]) * cRRI[18] - g[3][c]) * cRRI[19] -
+                           g[4][c]) * cRRI[20];
+                tmp = g[5][c] - tmp;
+                b[0][c] += tmp * cC[10];
+                b[1][c] += tmp * cC[11];
+                b[2][c] += tmp * cC[12];
+                b[3][c] += tmp * cC[13];
+                b[4][c] += tmp * cC[14];
+                b[5][c] += tmp;
+                break;
+            default:
+                tmp = g[6][c];
+                g[6][c] = ((((((gk * cRRI[21] - g[0][c]) * cRRI[22] - g[1][c]) * cRRI[23] - g[2][c]) * cRRI[24] - g[3][c]) * cRRI[25] -
+                            g[4][c]) * cRRI[26] - g[5][c]) * cRRI[27];
+                tmp = g[6][c] - tmp;
+                b[0][c] += tmp * cC[15];
+                b[1][c] += tmp * cC[16];
+                b[2][c] += tmp * cC[17];
+                b[3][c] += tmp * cC[18];
+                b[4][c] += tmp * cC[19];
+                b[5][c] += tmp * cC[20];
+                b[6][c] += tmp;
+                if (own) {
+                    const double ak = fabs(at[c]);
+                    if (is_normal(ak) && ak > maxak) maxak = ak;
+                    const double db = fabs(tmp);
+                    if (is_normal(db) && db > maxdb6) maxdb6 = db;
+                }
+                break;
+            }
+        }
+        if (n < 7) { n++; continue; }
+        // ---- end of a predictor-corrector iteration ----
+        const double pc_err = gmax<G>(maxdb6) / gmax<G>(maxak);
+        if (!(pc_err < 1e-16) && !(it > 2 && pc_err_last <= pc_err) && it < 12) {
+            pc_err_last = pc_err;
+            it++;
+            n = 1;
+            continue;
+        }
+        // ---- step-size control (real particle only) ----
+        double dt_new = dts;
+        if (io.epsilon > 0) {
+            double mak = 0.0, mb6 = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double ak = fabs(at[c]);
+                if (is_normal(ak) && ak > mak) mak = ak;
+                const double bk = fabs(b[6][c]);
+                if (is_normal(bk) && bk > mb6) mb6 = bk;
+            }
+            const double err = gbcast<G>(mb6 / mak, 0);
+            if (is_normal(err)) dt_new = pow(io.epsilon / err, 1.0 / 7.0) * dts;
+            else dt_new = dts / 0.25;
+            if (fabs(dt_new) < io.min_dt) dt_new = copysign(io.min_dt, dt_new);
+            if (fabs(dt_new / dts) < 0.25) {
+                rejected++;
+                dts = dt_new;
+                last = 0;
+                if (!setup_attempt(false)) finish();
+                continue;
+            }
+            if (dt_new / dts > 1. / 0.25) dt_new = dts / 0.25;
+        }
+        // ---- accept: advance x0, v0 (Kahan) and store br, er ----
+        const double dt2 = dts * dts;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            add_cs(x0[c], csx[c], b[6][c] / 72. * dt2);
+            add_cs(x0[c], csx[c], b[5][c] / 56. * dt2);
+            add_cs(x0[c], csx[c], b[4][c] / 42. * dt2);
+            add_cs(x0[c], csx[c], b[3][c] / 30. * dt2);
+            add_cs(x0[c], csx[c], b[2][c] / 20. * dt2);
+            add_cs(x0[c], csx[c], b[1][c] / 12. * dt2);
+            add_cs(x0[c], csx[c], b[0][c] / 6. * dt2);
+            add_cs(x0[c], csx[c], a0[c] / 2. * dt2);
+            add_cs(x0[c], csx[c], v0[c] * dts);
+            add_cs(v0[c], csv[c], b[6][c] / 8. * dts);
+            add_cs(v0[c], csv[c], b[5][c] / 7. * dts);
+            add_cs(v0[c], csv[c], b[4][c] / 6. * dts);
+            add_cs(v0[c], csv[c], b[3][c] / 5. * dts);
+            add_cs(v0[c], csv[c], b[2][c] / 4. * dts);
+            add_cs(v0[c], csv[c], b[1][c] / 3. * dts);
+            add_cs(v0[c], csv[c], b[0][c] / 2. * dts);
+            add_cs(v0[c], csv[c], a0[c] * dts);
+        }
+        if (own) {
+            const double q = (dt_last != 0.0) ? dts / dt_last : 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                double br[7], e[7];
+#pragma unroll
+                for (int k = 0; k < 7; k++) br[k] = scr[3 * k + c];
+                if (dt_last != 0.0 && q <= 20.0) predict_e(br, q, e);
+                else
+#pragma unroll
+                    for (int k = 0; k < 7; k++) e[k] = 0.0;
+#pragma unroll
+                for (int k = 0; k < 7; k++) { scr[21 + 3 * k + c] = e[k]; scr[3 * k + c] = b[k][c]; }
+            }
+        }
+        t = last ? t_end : t + dts;
+        steps++;
+        dt_last = dts;
+        if (!(last && io.epsilon <= 0)) dt = dt_new;
+        int bad = 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) bad |= !(fabs(x0[c]) < 1e300) || !(fabs(v0[c]) < 1e300);
+        bad = gor<G>(own ? bad : 0);
+        if (bad) { status = kNonFinite; finish(); continue; }
+        if (last) { finish(); continue; }
+        if (steps >= io.max_steps) { status = kMaxSteps; finish(); continue; }
+        phase = 0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// host launchers
+// ------------------------------------------------------------------------------------------
+template <int G, bool M>
+static hipError_t launch_forces_t(const ForcesArgs& a, hipStream_t s) {
+    const int threads = 256;
+    const long long total = (long long)a.nsys * G;
+    const int blocks = (int)((total + threads - 1) / threads);
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL((batch_forces_kernel<G, M>), dim3(blocks), dim3(threads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_forces(const ForcesArgs& a, int G, bool mixed, hipStream_t s) {
+    switch (G * 2 + (mixed ? 1 : 0)) {
+    case 2: return launch_forces_t<1, false>(a, s);
+    case 3: return launch_forces_t<1, true>(a, s);
+    case 4: return launch_forces_t<2, false>(a, s);
+    case 5: return launch_forces_t<2, true>(a, s);
+    case 8: return launch_forces_t<4, false>(a, s);
+    case 9: return launch_forces_t<4, true>(a, s);
+    case 16: return launch_forces_t<8, false>(a, s);
+    case 17: return launch_forces_t<8, true>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int G, bool M>
+static hipError_t launch_ias15_t(const IntegrateArgs& a, int blocks, int threads, hipStream_t s) {
+    hipLaunchKernelGGL((ias15_kernel<G, M>), dim3(blocks), dim3(threads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int blocks, int threads, hipStream_t s) {
+    switch (G * 2 + (mixed ? 1 : 0)) {
+    case 2: return launch_ias15_t<1, false>(a, blocks, threads, s);
+    case 3: return launch_ias15_t<1, true>(a, blocks, threads, s);
+    case 4: return launch_ias15_t<2, false>(a, blocks, threads, s);
+    case 5: return launch_ias15_t<2, true>(a, blocks, threads, s);
+    case 8: return launch_ias15_t<4, false>(a, blocks, threads, s);
+    case 9: return launch_ias15_t<4, true>(a, blocks, threads, s);
+    case 16: return launch_ias15_t<8, false>(a, blocks, threads, s);
+    case 17: return launch_ias15_t<8, true>(a, blocks, threads, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t ias15_occupancy(int G, bool mixed, int threads, int* blocks_per_cu) {
+    const void* f = nullptr;
+    switch (G * 2 + (mixed ? 1 : 0)) {
+    case 2: f = (const void*)ias15_kernel<1, false>; break;
+    case 3: f = (const void*)ias15_kernel<1, true>; break;
+    case 4: f = (const void*)ias15_kernel<2, false>; break;
+    case 5: f = (const void*)ias15_kernel<2, true>; break;
+    case 8: f = (const void*)ias15_kernel<4, false>; break;
+    case 9: f = (const void*)ias15_kernel<4, true>; break;
+    case 16: f = (const void*)ias15_kernel<8, false>; break;
+    case 17: f = (const void*)ias15_kernel<8, true>; break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, threads, 0);
+}
+
+// Sets the work-queue counter (a kernel, so the sequence can be captured in a graph).
+__global__ void set_counter_kernel(int* q, int v) { *q = v; }
+
+hipError_t launch_set_counter(int* q, int v, hipStream_t s) {
+    hipLaunchKernelGGL(set_counter_kernel, dim3(1), dim3(1), 0, s, q, v);
+    return hipGetLastError();
+}
+
+}  // namespace assist_gpu

@@ -492,9 +492,10 @@ def neo_like(model, n, rng, t0=0.0):
 
 
 def variational_identity(n, n_var=6):
-    """Initial variational vectors: the first ``n_var`` columns of the 6x6 identity."""
+    """Initial variational vectors: the first ``n_var`` columns of the 6x6 identity
+    (vectors beyond the sixth start at zero: they carry parameter variations)."""
     d = np.zeros((n, n_var, 6))
-    for k in range(n_var):
+    for k in range(min(n_var, 6)):
         d[:, k, k] = 1.0
     return d
 

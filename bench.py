@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Particle-days per second of the batched IAS15 + ASSIST force model on one MI355X.
+
+Workload (default): 262,144 systems -- 70% main-belt, 30% NEO-like, seed 4 --
+each a real test particle with 6 first-order variational particles, integrated
+365 days forward from JD 2451545.0 with IAS15 (epsilon 1e-9), all ASSIST
+forces on, on the seed-0 synthetic ephemeris.  One "step" is one integration
+of the whole batch; inputs are resident in HBM before the timed region.
+
+Prints one JSON line with the metric, the roofline of the integration kernel
+(algorithmic FLOPs from the operation count of the GPU algorithm divided by the
+kernel's HIP-event time) and the CPU baseline (the C oracle on the host cores,
+a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+# Populations with NEO-like orbits use min_dt = 0.001 d, as the reference does for encounters
+# (reference unit_tests/apophis_spk/problem.c:51): in a close planetary encounter the
+# round-off of the barycentric difference dominates IAS15's error estimate and would
+# otherwise drive the step to zero.
+CONFIGS = {
+    "c4": dict(n=262144, n_var=6, span=365.0, neo=0.3, seed=4, uneven=False, min_dt=0.001,
+               workload="262144 systems (70% main-belt, 30% NEO) x (1 + 6 variational), 365 d, eps 1e-9, "
+                        "min_dt 0.001 d, forces 0x7F"),
+    "c2": dict(n=4096, n_var=6, span=3650.0, neo=0.0, seed=2, uneven=False, min_dt=0.0,
+               workload="4096 main-belt systems x (1 + 6 variational), 3650 d, eps 1e-9, forces 0x7F"),
+    "c3": dict(n=65536, n_var=0, span=None, neo=0.3, seed=3, uneven=True, min_dt=0.001,
+               workload="65536 systems (70% main-belt, 30% NEO), log-uniform 30-10000 d spans, half backward, "
+                        "10% Marsden, eps 1e-9, min_dt 0.001 d"),
+}
+
+
+def opcounts():
+    """Operation counts of the GPU algorithm (assist_amd/csrc/opcount.cpp)."""
+    exe = os.path.join(ROOT, "assist_amd", "_build", "opcount")
+    src = os.path.join(ROOT, "assist_amd", "csrc", "opcount.cpp")
+    if not os.path.exists(exe) or os.path.getmtime(exe) < os.path.getmtime(src):
+        os.makedirs(os.path.dirname(exe), exist_ok=True)
+        subprocess.run(["g++", "-O1", "-std=c++17", "-o", exe, src], check=True)
+    return json.loads(subprocess.run([exe], check=True, capture_output=True, text=True).stdout)
+
+
+def algorithmic_flops(oc, nvec, ng, mixed, evals, steps, rejected):
+    """FP64 and FP32 operations of one launch from per-system eval/step counts; ``ng`` is the
+    per-system flag of a non-zero Marsden model."""
+    form = "mixed" if mixed else "fp64"
+    ev0 = oc["eval"]["nvec%d_ng0_%s" % (nvec, form)]
+    ev1 = oc["eval"].get("nvec%d_ng1_%s" % (nvec, form), ev0)
+    ias = oc["ias15"]
+    comps = 3 * nvec
+    ng = np.zeros(len(evals), dtype=bool) if ng is None else np.asarray(ng, dtype=bool)
+    ev_tot = float(np.sum(evals))
+    ev_ng = float(np.sum(evals[ng]))
+    st_tot = float(np.sum(steps))
+    rej_tot = float(np.sum(rejected)) if rejected is not None else 0.0
+    substeps = ev_tot - st_tot
+    f64 = (ev_tot - ev_ng) * ev0["fp64"] + ev_ng * ev1["fp64"]
+    f32 = (ev_tot - ev_ng) * ev0["fp32"] + ev_ng * ev1["fp32"]
+    f64 += substeps * (ias["substep_scalar"] + comps * (ias["predict_comp"] + ias["correct_comp_7substeps"] / 7.0))
+    f64 += (st_tot + rej_tot) * (comps * ias["setup_comp"] + ias["control"]) + st_tot * comps * ias["accept_comp"]
+    return f64, f32
+
+
+def make_workload(cfg, model):
+    from assist_amd import synthetic
+    rng = np.random.default_rng(cfg["seed"])
+    n, nv = cfg["n"], 1 + cfg["n_var"]
+    st, _ = synthetic.mixed_population(model, n, rng, cfg["neo"], 0.0)
+    states = np.zeros((n, nv, 6))
+    states[:, 0] = st
+    if cfg["n_var"]:
+        states[:, 1:] = synthetic.variational_identity(n, cfg["n_var"])
+    t0 = np.zeros(n)
+    params = None
+    if cfg["uneven"]:
+        t1 = synthetic.log_uniform_spans(n, rng)
+        params = np.zeros((n, nv, 3))
+        params[:, 0] = synthetic.marsden_params(n, rng, 0.1)
+    else:
+        t1 = np.full(n, cfg["span"])
+    return t0, t1, states, params
+
+
+def cpu_baseline(paths, cfg, t0, t1, states, params, target_cpu_s=20.0):
+    """The C oracle on the host cores over a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    o = Oracle(*paths)
+    cores = min(os.cpu_count() or 1, 16)
+    probe = 16
+    tp = time.perf_counter()
+    o.integrate(t0[:probe], t1[:probe], states[:probe], None if params is None else params[:probe], nthreads=1,
+                cfg=o.cfg(min_dt=cfg["min_dt"]))
+    per_sys = (time.perf_counter() - tp) / probe
+    m = int(min(len(t0), max(cores, target_cpu_s / max(per_sys, 1e-6))))
+    sel = slice(0, m)
+    ts = time.perf_counter()
+    r = o.integrate(t0[sel], t1[sel], states[sel], None if params is None else params[sel], nthreads=cores,
+                    cfg=o.cfg(min_dt=cfg["min_dt"]))
+    el = time.perf_counter() - ts
+    pdays = float(np.sum(np.abs(t1[sel] - t0[sel])))
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(value=pdays / el, unit="particle-days/s", cores=cores, kind="port",
+                sample="first %d systems of the workload (%.0f particle-days) on %d threads, %s" % (m, pdays, cores, cpu_model),
+                seconds=el), r, m
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--n", type=int, default=0, help="override the number of systems")
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: every system is independent, so N GPUs would each take 1/N of the "
+              "batch; this build measures the single-GPU shard (run with --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import assist_amd
+    from assist_amd import batch, synthetic
+    if batch.gpu_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    cfg = dict(CONFIGS[args.config])
+    if args.n:
+        cfg["n"] = args.n
+    paths = synthetic.ensure_synthetic_ephemeris(0)
+    ephem = assist_amd.Ephem(*paths)
+    model = synthetic.SyntheticModel(0)
+    t0, t1, states, params = make_workload(cfg, model)
+    n, nv = states.shape[0], states.shape[1]
+    mixed = args.precision == "mixed"
+    bc = batch.BatchConfig.default(precision=1 if mixed else 0, lanes_per_system=args.lanes, block_size=args.block,
+                                   min_dt=cfg["min_dt"], max_steps=200000)
+
+    d_t0, d_t1 = batch.DeviceArray(t0), batch.DeviceArray(t1)
+    d_init = batch.DeviceArray(states)
+    d_states = batch.DeviceArray(states)
+    d_params = batch.DeviceArray(params) if params is not None else None
+    d_status = batch.DeviceArray(np.zeros(n, dtype=np.int32))
+    d_steps = batch.DeviceArray(np.zeros(n, dtype=np.int64))
+    d_evals = batch.DeviceArray(np.zeros(n, dtype=np.int64))
+
+    def one_step():
+        d_states.copy_from(d_init)
+        return batch.integrate_device(ephem, n, nv - 1, d_t0, d_t1, d_states, d_params, d_status, d_steps, d_evals,
+                                      config=bc)
+
+    for _ in range(args.warmup):
+        one_step()
+        print("warmup step done", file=sys.stderr, flush=True)
+    batch.synchronize()
+    kernel_ms = []
+    ts = time.perf_counter()
+    for k in range(args.steps):
+        st = one_step()
+        kernel_ms.append(st.kernel_ms)
+        print("step %d: kernel %.1f ms" % (k, st.kernel_ms), file=sys.stderr, flush=True)
+    batch.synchronize()
+    elapsed = time.perf_counter() - ts
+
+    status = d_status.download()
+    steps = d_steps.download()
+    evals = d_evals.download()
+    out_states = d_states.download()
+    pdays = float(np.sum(np.abs(t1 - t0)))
+    value = pdays * args.steps / elapsed
+    oc = opcounts()
+    ng = None if params is None else np.any(params[:, 0] != 0, axis=-1)
+    f64, f32 = algorithmic_flops(oc, nv, ng, mixed, evals, steps, None)
+    kms = float(np.mean(kernel_ms))
+    ach64 = f64 / (kms * 1e-3) / 1e12
+    ach32 = f32 / (kms * 1e-3) / 1e12
+    frac = ach64 / FP64_PEAK_TFLOPS + ach32 / FP32_PEAK_TFLOPS
+    result = {
+        "metric": "particle-days/sec (plain FP64 vs mixed+learned) at <=1 m 10-yr position error",
+        "value": value,
+        "unit": "particle-days/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if not mixed else "f64+f32",
+        "data": "synthetic ephemeris (seed 0) and synthetic particle states",
+        "config": {"workload": cfg["workload"], "systems": n, "vectors_per_system": nv, "form": args.precision,
+                   "particle_days_per_step": pdays, "lanes_per_system": st.lanes_per_system,
+                   "block_size": st.block_size, "grid_blocks": st.grid_blocks},
+        "roofline": {"bound": "fp64-valu", "achieved": ach64 + ach32, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": frac, "traffic": None,
+                     "kernel": "ias15_kernel", "kernel_ms": kms, "flops_fp64_per_launch": f64,
+                     "flops_fp32_per_launch": f32, "achieved_fp64": ach64, "achieved_fp32": ach32},
+        "integration": {"steps_total": int(steps.sum()), "evals_total": int(evals.sum()),
+                        "status_nonzero": int((status != 0).sum())},
+    }
+    if not args.no_cpu_baseline:
+        cb, r, m = cpu_baseline(paths, cfg, t0, t1, states, params)
+        dpos = np.linalg.norm(out_states[:m, 0, :3] - r["states"][:, 0, :3], axis=-1) * 149597870700.0
+        cb.pop("seconds", None)
+        result["cpu_baseline"] = cb
+        result["parity_vs_oracle_sample"] = {"systems": m, "max_position_diff_m": float(dpos.max())}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,301 @@
+/*
+ * assist-amd public C ABI (libassist.so).
+ *
+ * Part 1 keeps the reference's API surface (reference src/assist.h:36-304,
+ * src/forces.h:30-36): the same enums, the same `struct assist_ephem` and
+ * `struct assist_extras` field layouts (the Python wrapper mirrors them with
+ * ctypes, reference assist/ephem.py:95-120 and assist/extras.py:84-100), the
+ * same function names, argument meanings and error conventions.
+ *
+ * REBOUND, which hosts the reference as a plugin, is not part of this build;
+ * `struct reb_simulation` / `struct reb_particle` below are a minimal
+ * project-defined simulation carrying exactly the fields the force model and
+ * the drop-in entry points use.
+ *
+ * Part 2 adds batched entry points: many independent systems (one real test
+ * particle plus its first-order variational particles), each with its own
+ * time, IAS15 step, span and direction, evaluated/integrated on the GPU.
+ */
+#ifndef ASSIST_AMD_ASSIST_H
+#define ASSIST_AMD_ASSIST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------------
+ * Part 1: reference API surface
+ * ---------------------------------------------------------------------------------------- */
+
+extern const char* assist_build_str;    /* reference src/assist.h:36 */
+extern const char* assist_version_str;  /* reference src/assist.h:37 */
+extern const char* assist_githash_str;  /* reference src/assist.h:38 */
+extern const char* assist_error_messages[];  /* reference src/assist.c:52-59 */
+extern const int assist_error_messages_N;    /* reference src/assist.c:60 */
+
+/* reference src/assist.h:52-57 */
+typedef enum {
+    FILE_FORMAT_VALID_BSP = 0,
+    FILE_FORMAT_ASCII_BIN = 2,
+    FILE_FORMAT_UNKNOWN = 3
+} ephemeris_file_format_t;
+
+/* reference src/assist.h:76-87 */
+enum ASSIST_FORCES {
+    ASSIST_FORCE_NONE = 0,
+    ASSIST_FORCE_SUN = 0x01,
+    ASSIST_FORCE_PLANETS = 0x02,
+    ASSIST_FORCE_ASTEROIDS = 0x04,
+    ASSIST_FORCE_NON_GRAVITATIONAL = 0x08,
+    ASSIST_FORCE_EARTH_HARMONICS = 0x10,
+    ASSIST_FORCE_SUN_HARMONICS = 0x20,
+    ASSIST_FORCE_GR_EIH = 0x40,
+    ASSIST_FORCE_GR_SIMPLE = 0x80,
+    ASSIST_FORCE_GR_POTENTIAL = 0x100,
+};
+
+/* reference src/assist.h:90-98 */
+enum ASSIST_STATUS {
+    ASSIST_SUCCESS,
+    ASSIST_ERROR_EPHEM_FILE,
+    ASSIST_ERROR_AST_FILE,
+    ASSIST_ERROR_NAST,
+    ASSIST_ERROR_NEPHEM,
+    ASSIST_ERROR_COVERAGE,
+    ASSIST_ERROR_N,
+};
+
+/* reference src/assist.h:102-116 */
+enum ASSIST_BODY {
+    ASSIST_BODY_SUN = 0,
+    ASSIST_BODY_MERCURY = 1,
+    ASSIST_BODY_VENUS = 2,
+    ASSIST_BODY_EARTH = 3,
+    ASSIST_BODY_MOON = 4,
+    ASSIST_BODY_MARS = 5,
+    ASSIST_BODY_JUPITER = 6,
+    ASSIST_BODY_SATURN = 7,
+    ASSIST_BODY_URANUS = 8,
+    ASSIST_BODY_NEPTUNE = 9,
+    ASSIST_BODY_PLUTO = 10,
+    ASSIST_BODY_NPLANETS = 11,
+};
+
+struct ascii_s;
+struct spk_s;
+
+/* Same field order as reference src/assist.h:125-155. */
+struct assist_ephem {
+    double jd_ref;
+    struct spk_s* spk_planets;
+    struct spk_s* spk_asteroids;
+    struct ascii_s* ascii_planets;
+    int planets_source;
+    int (*planets_calc)(const struct assist_ephem*, double, double, int, double* const, double* const,
+                        double* const, double* const, double* const, double* const, double* const,
+                        double* const, double* const, double* const);
+    int spk_target_index[ASSIST_BODY_NPLANETS];
+    int spk_emb_index;
+    double AU;
+    double EMRAT;
+    double J2E;
+    double J3E;
+    double J4E;
+    double J2SUN;
+    double RE;
+    double CLIGHT;
+    double ASUN;
+    double Re_eq;
+    double Rs_eq;
+    double c_AU_per_day;
+    double c_squared;
+    double over_c_squared;
+};
+
+/* reference src/assist.h:157-175 */
+struct assist_cache_item {
+    double GM, x, y, z, vx, vy, vz, ax, ay, az;
+};
+struct assist_ephem_cache {
+    double* t;
+    double dt_sign;
+    struct assist_cache_item* items;
+};
+
+/* Minimal simulation (REBOUND is not part of this build). */
+struct reb_particle {
+    double x, y, z, vx, vy, vz, ax, ay, az;
+    double m;
+};
+struct reb_variational_configuration {
+    int index;        /* particle index of the variational particle */
+    int testparticle; /* index of the real particle it belongs to */
+};
+struct reb_simulation_integrator_ias15 {
+    double epsilon;   /* default 1e-9; 0 selects a fixed step */
+    double min_dt;
+    int adaptive_mode; /* 1 (global criterion) is the only mode; set by assist_init */
+};
+struct reb_simulation {
+    double t;
+    double dt;
+    double dt_last_done;
+    int N;            /* real + variational particles */
+    int N_var;
+    int N_var_config;
+    int allocated_N;
+    struct reb_particle* particles;
+    struct reb_variational_configuration* var_config;
+    int status;       /* 0 running/success, >0 error (reference REB_STATUS_GENERIC_ERROR semantics) */
+    int exact_finish_time;
+    struct reb_simulation_integrator_ias15 ri_ias15;
+    void* extras;
+    void (*additional_forces)(struct reb_simulation*);
+    char error_message[256];
+};
+
+/* Same field order as reference src/assist.h:177-195. */
+struct assist_extras {
+    struct reb_simulation* sim;
+    struct assist_ephem* ephem;
+    struct assist_ephem_cache* ephem_cache;
+    int extras_should_free_ephem;
+    int geocentric;
+    struct reb_particle* last_state;
+    struct reb_particle* current_state;
+    double* particle_params;
+    int steps_done;
+    int forces;
+    int gr_eih_sources;
+    double alpha;
+    double nk;
+    double nm;
+    double nn;
+    double r0;
+};
+
+/* Ephemeris lifecycle and queries -- reference src/assist.h:271-300, src/assist.c:188-554 */
+struct assist_ephem* assist_ephem_create(char* planets_file_name, char* asteroids_file_name);
+int assist_ephem_init(struct assist_ephem* ephem, char* user_planets_path, char* user_asteroids_path);
+void assist_ephem_free(struct assist_ephem* ephem);
+void assist_ephem_free_pointers(struct assist_ephem* ephem);
+void assist_ephem_time_bounds(const struct assist_ephem* ephem, double* t_beg, double* t_end);
+struct reb_particle assist_get_particle(const struct assist_ephem* ephem, const int particle_id, const double t);
+struct reb_particle assist_get_particle_with_error(const struct assist_ephem* ephem, const int particle_id,
+                                                   const double t, int* error);
+/* reference src/forces.h:32-36 (cache may be NULL) */
+int assist_all_ephem(const struct assist_ephem* ephem, struct assist_ephem_cache* cache, const int i, const double t,
+                     double* const GM, double* const x, double* const y, double* const z, double* const vx,
+                     double* const vy, double* const vz, double* const ax, double* const ay, double* const az);
+
+/* Format detection and discovery -- reference src/assist.h:63-70, src/assist.c:66-154 */
+int assist_detect_ascii_bin_signature(int fd);
+ephemeris_file_format_t assist_detect_ephemeris_file_format(int fd);
+int assist_discover_planets_path(char* out_path, size_t out_path_size, const char* assist_dir);
+
+/* Extras lifecycle -- reference src/assist.h:197-229, src/assist.c:379-501 */
+struct assist_extras* assist_attach(struct reb_simulation* sim, struct assist_ephem* ephem);
+void assist_init(struct assist_extras* assist, struct reb_simulation* sim, struct assist_ephem* ephem);
+void assist_detach(struct reb_simulation* sim, struct assist_extras* assist);
+void assist_free(struct assist_extras* assist);
+void assist_free_pointers(struct assist_extras* assist);
+void assist_error(struct assist_extras* assist, const char* const msg);
+
+/* The force hook -- reference src/forces.h:30 (assist_additional_forces, installed as
+ * REBOUND's additional_forces at src/assist.c:444).  Evaluated on the GPU. */
+void assist_additional_forces(struct reb_simulation* sim);
+
+/* Minimal simulation API (stands in for REBOUND's). */
+struct reb_simulation* reb_simulation_create(void);
+void reb_simulation_init(struct reb_simulation* sim);          /* for caller-allocated structs */
+void reb_simulation_free(struct reb_simulation* sim);
+void reb_simulation_free_pointers(struct reb_simulation* sim); /* frees particles, keeps the struct */
+void reb_simulation_add(struct reb_simulation* sim, struct reb_particle p);
+int reb_simulation_add_variation_1st_order(struct reb_simulation* sim, int testparticle);
+/* Integrates with IAS15 on the GPU to tmax (exact finish). */
+void reb_simulation_integrate(struct reb_simulation* sim, double tmax);
+void reb_simulation_error(struct reb_simulation* sim, const char* msg);
+
+/* ------------------------------------------------------------------------------------------
+ * Part 2: batched GPU entry points (no counterpart in the reference)
+ * ---------------------------------------------------------------------------------------- */
+
+enum ASSIST_BATCH_ERROR {
+    ASSIST_BATCH_OK = 0,
+    ASSIST_BATCH_ERROR_ARG = 100,      /* invalid argument */
+    ASSIST_BATCH_ERROR_DEVICE = 101,   /* HIP runtime error or no GPU */
+    ASSIST_BATCH_ERROR_UNSUPPORTED = 102, /* option not supported on the GPU path */
+};
+
+/* Per-system status codes written by the batched integrator. */
+enum ASSIST_SYSTEM_STATUS {
+    ASSIST_SYSTEM_OK = 0,
+    ASSIST_SYSTEM_COVERAGE = 5,      /* == ASSIST_ERROR_COVERAGE */
+    ASSIST_SYSTEM_NONFINITE = -1,
+    ASSIST_SYSTEM_MAX_STEPS = -2,
+    ASSIST_SYSTEM_DT_UNDERFLOW = -3,
+};
+
+enum ASSIST_PRECISION { ASSIST_PRECISION_FP64 = 0, ASSIST_PRECISION_MIXED = 1 };
+
+struct assist_batch_config {
+    double epsilon;       /* IAS15 epsilon (default 1e-9); 0 = fixed step */
+    double min_dt;        /* default 0 */
+    double dt_initial;    /* first trial step in days (default 0.001) */
+    long long max_steps;  /* per system (default 10^6) */
+    int precision;        /* ASSIST_PRECISION_* */
+    int lanes_per_system; /* 0 = choose (autotuned), else 1, 2, 4 or 8 (>= 1 + n_var) */
+    int block_size;       /* 0 = choose, else a multiple of 64 up to 256 */
+    int blocks_per_cu;    /* 0 = occupancy limit */
+};
+
+struct assist_batch_stats {
+    long long steps, rejected, evals;
+    double kernel_ms;     /* device time of the integration kernel (HIP events) */
+    int lanes_per_system, block_size, grid_blocks;
+};
+
+void assist_batch_config_default(struct assist_batch_config* cfg);
+const char* assist_batch_error_string(int code);
+/* Number of visible GPUs (0 when none / runtime unavailable). */
+int assist_gpu_device_count(void);
+
+/* Force model for n systems at their own times t[i].  `opts` carries forces, gr_eih_sources,
+ * geocentric and the Marsden shape (alpha, nk, nm, nn, r0); NULL selects the defaults of
+ * assist_init.  states: n x (1+n_var) x 6 (x,y,z,vx,vy,vz; real particle first), params:
+ * n x (1+n_var) x 3 (A1,A2,A3 for the real particle, their variations for the variational
+ * ones) or NULL.  acc: n x (1+n_var) x 3.  status: per system (ASSIST_ERROR_COVERAGE etc.). */
+int assist_batch_forces(const struct assist_ephem* ephem, const struct assist_extras* opts, int n, int n_var,
+                        const double* t, const double* states, const double* params, double* acc, int* status,
+                        int precision);
+
+/* Integrates n systems with IAS15, system i from t0[i] to t1[i] (either direction), states
+ * updated in place.  Host pointers; copies to and from the GPU are included. */
+int assist_batch_integrate(const struct assist_ephem* ephem, const struct assist_extras* opts,
+                           const struct assist_batch_config* cfg, int n, int n_var, const double* t0,
+                           const double* t1, double* states, const double* params, int* status,
+                           long long* steps, struct assist_batch_stats* stats);
+
+/* Same with device pointers (inputs already resident in HBM); `stream` is a hipStream_t or
+ * NULL.  Returns after the kernel completes; stats->kernel_ms is the kernel's device time. */
+int assist_batch_integrate_device(const struct assist_ephem* ephem, const struct assist_extras* opts,
+                                  const struct assist_batch_config* cfg, int n, int n_var, const double* d_t0,
+                                  const double* d_t1, double* d_states, const double* d_params, int* d_status,
+                                  long long* d_steps, long long* d_evals, void* stream,
+                                  struct assist_batch_stats* stats);
+
+/* Device buffer helpers for callers without their own allocator. */
+void* assist_device_malloc(size_t bytes);
+void assist_device_free(void* p);
+int assist_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int assist_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int assist_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int assist_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,208 @@
+"""GPU parity against the CPU oracle (HIP kernels through the C-ABI of libassist.so).
+
+Bars (from the hot-path contract):
+  * FP64 force evaluation: relative error <= 1e-12 per evaluated vector, for the
+    full model and for every term on its own;
+  * FP64 integration: real-particle position within 1 m of the oracle after the
+    span for orbits without close approaches (10 years for the main belt).
+"""
+import numpy as np
+import pytest
+
+from conftest import AU_M, population, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TERMS = {
+    "SUN": 0x01, "PLANETS": 0x02, "ASTEROIDS": 0x04, "NON_GRAVITATIONAL": 0x08, "EARTH_HARMONICS": 0x10,
+    "SUN_HARMONICS": 0x20, "GR_EIH": 0x40, "GR_SIMPLE": 0x80, "GR_POTENTIAL": 0x100,
+}
+
+
+def _forces_both(ephem, oracle, t, states, params, forces, precision=0):
+    from assist_amd import batch
+    opts = dict(forces=forces)
+    g, gs = batch.forces(ephem, t, states, params, extras=opts, precision=precision)
+    o, os_ = oracle.forces(t, states, params, oracle.opts(forces=forces))
+    assert (gs == os_).all()
+    return g, o
+
+
+def _sample(model, n, seed, n_var=6, ng=0.5):
+    rng = np.random.default_rng(seed)
+    states, params, _ = population(model, n, seed, n_var=n_var, ng_fraction=ng)
+    t = rng.uniform(-9500.0, 9500.0, n)
+    if n_var:
+        # generic variational vectors, and Marsden-parameter variations on the last one
+        states[:, 1:] = rng.normal(size=states[:, 1:].shape)
+        params[:, n_var] = rng.normal(size=(n, 3)) * 1e-12
+    return t, states, params
+
+
+def test_forces_full_model_fp64(gpu, ephem, oracle, model):
+    t, states, params = _sample(model, 512, 11)
+    g, o = _forces_both(ephem, oracle, t, states, params, 0x7F)
+    e_real = rel_err(g[:, 0], o[:, 0])
+    e_var = rel_err(g[:, 1:], o[:, 1:])
+    print("full model: max rel err real %.3e var %.3e" % (e_real.max(), e_var.max()))
+    assert e_real.max() <= 1e-12
+    assert e_var.max() <= 1e-12
+
+
+@pytest.mark.parametrize("term", list(FORCE_TERMS))
+def test_forces_each_term_fp64(gpu, ephem, oracle, model, term):
+    t, states, params = _sample(model, 256, 12)
+    g, o = _forces_both(ephem, oracle, t, states, params, FORCE_TERMS[term])
+    e_real = rel_err(g[:, 0], o[:, 0])
+    e_var = rel_err(g[:, 1:], o[:, 1:])
+    nz = np.linalg.norm(o[:, 0], axis=-1) > 0
+    print("%s: max rel err real %.3e var %.3e" % (term, e_real[nz].max() if nz.any() else 0, e_var.max()))
+    assert np.abs(g[~nz, 0]).max(initial=0.0) == 0.0
+    assert e_real.max() <= 1e-12
+    assert e_var.max() <= 1e-12
+
+
+@pytest.mark.parametrize("n_var", [0, 1, 3, 7])
+def test_forces_variational_counts(gpu, ephem, oracle, model, n_var):
+    t, states, params = _sample(model, 128, 13, n_var=n_var)
+    g, o = _forces_both(ephem, oracle, t, states, params, 0x7F)
+    assert rel_err(g, o).max() <= 1e-12
+
+
+def test_forces_coverage_status(gpu, ephem, oracle, model):
+    from assist_amd import batch
+    states, _, _ = population(model, 4, 3, n_var=0)
+    t = np.array([0.0, 10001.0, -10001.0, 9999.0])
+    _, st = batch.forces(ephem, t, states)
+    _, ost = oracle.forces(t, states)
+    assert list(st) == [0, 5, 5, 0]
+    assert list(ost) == [0, 5, 5, 0]
+
+
+def test_forces_mixed_precision_error(gpu, ephem, oracle, model):
+    """Mixed form: real acceleration stays at FP64 level; variational partials are FP32."""
+    t, states, params = _sample(model, 512, 14)
+    g, o = _forces_both(ephem, oracle, t, states, params, 0x7F, precision=1)
+    e_real = rel_err(g[:, 0], o[:, 0]).max()
+    e_var = rel_err(g[:, 1:], o[:, 1:]).max()
+    print("mixed: max rel err real %.3e var %.3e" % (e_real, e_var))
+    assert e_real <= 1e-11
+    assert e_var <= 1e-5
+
+
+def _integrate_both(ephem, oracle, t0, t1, states, params=None, **cfg):
+    from assist_amd import batch
+    r = batch.integrate(ephem, t0, t1, states, params, config=batch.BatchConfig.default(**cfg))
+    ocfg = oracle.cfg(**{k: v for k, v in cfg.items() if k in ("epsilon", "min_dt")})
+    if "dt_initial" in cfg:
+        ocfg.dt = cfg["dt_initial"]
+    o = oracle.integrate(t0, t1, states, params, cfg=ocfg)
+    return r, o
+
+
+def test_integrate_main_belt_10yr_with_variations(gpu, ephem, oracle, model):
+    """Configuration-2 shape at reduced count: 64 main-belt systems x (1+6), 3650 days."""
+    from assist_amd import synthetic
+    rng = np.random.default_rng(2)
+    n = 64
+    states = np.zeros((n, 7, 6))
+    states[:, 0] = synthetic.main_belt(model, n, rng)
+    states[:, 1:] = synthetic.variational_identity(n, 6)
+    r, o = _integrate_both(ephem, oracle, 0.0, 3650.0, states)
+    assert (r.status == 0).all() and (o["status"] == 0).all()
+    dpos = np.linalg.norm(r.states[:, 0, :3] - o["states"][:, 0, :3], axis=-1) * AU_M
+    dvar = rel_err(r.states[:, 1:], o["states"][:, 1:])
+    print("10 yr: max |dr| = %.3e m, steps gpu %d oracle %d, var rel %.3e" % (
+        dpos.max(), r.steps.sum(), o["steps"].sum(), dvar.max()))
+    assert dpos.max() <= 1.0
+    assert dvar.max() <= 1e-6
+    assert np.abs(r.steps - o["steps"]).max() <= 2
+
+
+def test_integrate_uneven_spans_both_directions(gpu, ephem, oracle, model):
+    """Configuration-3 shape at reduced count: mixed population, log-uniform spans, half backward, 10% NG."""
+    from assist_amd import synthetic
+    rng = np.random.default_rng(3)
+    n = 96
+    states, params, is_neo = population(model, n, 3, n_var=0, ng_fraction=0.1)
+    span = synthetic.log_uniform_spans(n, rng, 30.0, 3000.0)
+    r, o = _integrate_both(ephem, oracle, 0.0, span, states, params)
+    assert (r.status == 0).all() and (o["status"] == 0).all()
+    dpos = np.linalg.norm(r.states[:, 0, :3] - o["states"][:, 0, :3], axis=-1) * AU_M
+    # orbits that pass close to a planet amplify round-off; compare those on a relative scale
+    print("uneven: max |dr| main belt %.3e m, NEO %.3e m" % (dpos[~is_neo].max(), dpos[is_neo].max()))
+    assert dpos[~is_neo].max() <= 1.0
+    assert np.median(dpos[is_neo]) <= 1.0
+
+
+@pytest.mark.parametrize("mode,span,tol_m", [
+    ("fixed", 100.0, 1e-4), ("fixed", 1000.0, 1e-3), ("fixed", 10000.0, 5e-3),
+    ("adaptive", 100.0, 1e-2), ("adaptive", 1000.0, 5e-2), ("adaptive", 10000.0, 0.1)])
+def test_integrate_round_trip(gpu, ephem, oracle, model, mode, span, tol_m):
+    """Out and back (reference unit_tests/roundtrip_spk: fixed 10-day steps with epsilon = 0,
+    and roundtrip_adaptive_spk), for a main-belt orbit like the reference's, with the reference's
+    tolerances; the GPU round trip also matches the oracle's."""
+    from assist_amd import batch, synthetic
+    x = synthetic.config1_particle(model).reshape(1, 1, 6)
+    kw = dict(epsilon=0.0, dt_initial=10.0) if mode == "fixed" else {}
+    cfg = batch.BatchConfig.default(**kw)
+    out = batch.integrate(ephem, 0.0, span, x, config=cfg)
+    back = batch.integrate(ephem, span, 0.0, out.states, config=cfg)
+    d = np.linalg.norm(back.states[0, 0, :3] - x[0, 0, :3]) * AU_M
+    ocfg = dict(epsilon=0.0, dt=10.0) if mode == "fixed" else {}
+    oo = oracle.integrate(0.0, span, x, cfg=oracle.cfg(**ocfg))
+    ob = oracle.integrate(span, 0.0, oo["states"], cfg=oracle.cfg(**ocfg))
+    do = np.linalg.norm(ob["states"][0, 0, :3] - x[0, 0, :3]) * AU_M
+    dgo = np.linalg.norm(back.states[0, 0, :3] - ob["states"][0, 0, :3]) * AU_M
+    print("%s round trip %g d: gpu %.3e m, oracle %.3e m, gpu-oracle %.3e m" % (mode, span, d, do, dgo))
+    assert d < tol_m and do < tol_m
+    assert dgo < tol_m
+
+
+def test_integrate_edge_cases(gpu, ephem, model):
+    from assist_amd import batch
+    states, _, _ = population(model, 4, 5, n_var=0)
+    r = batch.integrate(ephem, np.array([0.0, 0.0, 0.0, 9990.0]), np.array([0.0, 10.0, -10.0, 10020.0]), states)
+    assert r.status[0] == 0 and r.steps[0] == 0
+    assert np.array_equal(r.states[0], states[0])
+    assert r.status[1] == 0 and r.status[2] == 0
+    assert r.status[3] == 5  # leaves the ephemeris coverage
+    empty = batch.integrate(ephem, np.zeros(0), np.zeros(0), np.zeros((0, 1, 6)))
+    assert empty.states.shape == (0, 1, 6)
+
+
+def test_variational_matches_shadow_particle(gpu, ephem, model):
+    """Variational particle vs a shadow particle displaced by 10 m (reference unit_tests/variational_spk)."""
+    from assist_amd import batch, synthetic
+    x = synthetic.config1_particle(model)
+    dx = 10.0 / AU_M
+    states = np.zeros((2, 2, 6))
+    states[:, 0] = x
+    states[1, 0, 0] += dx
+    states[:, 1, 0] = 1.0
+    for span in (1.0, 101.0):
+        r = batch.integrate(ephem, 0.0, span, states, config=batch.BatchConfig.default(lanes_per_system=2))
+        diff1 = r.states[1, 0, 0] - r.states[0, 0, 0]
+        diff2 = r.states[0, 1, 0] * dx
+        rel = abs((diff1 - diff2) / diff1)
+        print("span %g: relative difference %.3e" % (span, rel))
+        assert rel < 1e-5
+
+
+def test_simulation_dropin_matches_oracle(gpu, ephem, oracle, model):
+    """Reference-style use: Simulation + Extras, integrate, compare with the oracle."""
+    import assist_amd
+    from assist_amd import synthetic
+    x = synthetic.config1_particle(model)
+    sim = assist_amd.Simulation()
+    ex = assist_amd.Extras(sim, ephem)
+    sim.t = 0.0
+    sim.add(x=x[0], y=x[1], z=x[2], vx=x[3], vy=x[4], vz=x[5])
+    sim.integrate(1000.0)
+    o = oracle.integrate(0.0, 1000.0, x.reshape(1, 1, 6))
+    p = sim.particles[0]
+    d = np.linalg.norm(np.array([p.x, p.y, p.z]) - o["states"][0, 0, :3]) * AU_M
+    print("config 1 (1000 d): |dr| = %.3e m, t = %g" % (d, sim.t))
+    assert sim.t == 1000.0
+    assert d <= 1.0
+    del ex
