@@ -52,7 +52,8 @@ bool hip_ok(hipError_t e, const char* what) {
     return false;
 }
 
-// Flattens one SPK target into equal-length records [jd_ref-jul(MID), 86400/RADIUS, coeffs, pad].
+// Flattens one SPK target into records [jd_ref-jul(MID), 86400/RADIUS, x, y, z coefficients],
+// each coordinate zero-padded to kCoefPad coefficients.
 // Returns false when the target's segments are not contiguous with equal record length.
 bool flatten_target(const spk_s* s, const spk_target& t, double jd_ref, std::vector<double>& out, DevSlot& slot,
                     size_t& offset_words) {
@@ -71,10 +72,9 @@ bool flatten_target(const spk_s* s, const spk_target& t, double jd_ref, std::vec
         nrec_total += N;
     }
     const int P = (rsize - 2) / 3;
-    if (P < 2 || P > kMaxCoef) return false;
-    const int Rd = ((2 + 3 * P) + 1) & ~1;  // even number of doubles
+    if (P < 2 || P > kCoefPad) return false;
     offset_words = out.size();
-    out.resize(out.size() + (size_t)nrec_total * Rd, 0.0);
+    out.resize(out.size() + (size_t)nrec_total * kRecWords, 0.0);
     double* dst = out.data() + offset_words;
     for (size_t k = 0; k < t.two.size(); k++) {
         const double* dir = base + t.two[k] - 1;
@@ -84,8 +84,8 @@ bool flatten_target(const spk_s* s, const spk_target& t, double jd_ref, std::vec
             const double* rec = src + (size_t)r * rsize;
             dst[0] = jd_ref - jul(rec[0]);
             dst[1] = 1.0 / (rec[1] / 86400.0);
-            memcpy(dst + 2, rec + 2, sizeof(double) * 3 * P);
-            dst += Rd;
+            for (int c = 0; c < 3; c++) memcpy(dst + 2 + c * kCoefPad, rec + 2 + c * P, sizeof(double) * P);
+            dst += kRecWords;
         }
     }
     slot.rec = nullptr;
@@ -93,7 +93,6 @@ bool flatten_target(const spk_s* s, const spk_target& t, double jd_ref, std::vec
     slot.inv_intlen = 1.0 / (intlen / 86400.0);
     slot.nrec = nrec_total;
     slot.P = P;
-    slot.R = Rd;
     return true;
 }
 

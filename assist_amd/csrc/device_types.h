@@ -3,8 +3,10 @@
 // The SPK type-2 data of the 28 Chebyshev targets the force model needs are
 // flattened into one device buffer.  Each target ("slot") is a contiguous run
 // of equal-length records; a record holds
-//     [ jd_ref - jul(MID),  86400/RADIUS,  x coeffs (P), y coeffs (P), z coeffs (P), pad ]
-// padded to an even number of doubles so 16-byte vector loads stay aligned.
+//     [ jd_ref - jul(MID),  86400/RADIUS,  x coeffs, y coeffs, z coeffs ]
+// with every coordinate zero-padded to kCoefPad coefficients, so that all
+// records have one size and one fully unrolled evaluation (a zero coefficient
+// adds an exact zero).
 // The first two words replace the SPK record's MID and RADIUS with the
 // quantities the evaluation actually uses (reference src/spk.c:445,514:
 // z = ((jd_ref - jul(MID)) + t) / (RADIUS/86400)).
@@ -22,7 +24,8 @@ namespace assist_gpu {
 constexpr int kNumSlots = 28;       // 12 planet-file targets + 16 asteroids
 constexpr int kNumBodies = 27;      // ASSIST body indices 0..26
 constexpr int kNumPlanets = 11;     // ASSIST_BODY_NPLANETS
-constexpr int kMaxCoef = 16;
+constexpr int kCoefPad = 14;                 // coefficients per coordinate on the device
+constexpr int kRecWords = 2 + 3 * kCoefPad;  // 44 doubles = 352 bytes per record
 constexpr int kSlotEMB = 3;         // Earth-Moon barycentre (no force of its own)
 constexpr int kSlotEarth = 4;       // Earth relative to the EMB
 constexpr int kSlotMoon = 5;        // Moon relative to the EMB
@@ -40,9 +43,9 @@ struct DevSlot {
     double c_init;       // jd_ref - jul(INIT) of the first record
     double inv_intlen;   // 86400 / INTLEN  (1/record length in days)
     int nrec;
-    int P;               // coefficients per coordinate
-    int R;               // record stride in doubles (padded)
+    int P;               // coefficients per coordinate in the file (<= kCoefPad)
     int kind;            // 0 planet-file target, 1 asteroid
+    int pad_;
 };
 
 struct DevEphem {

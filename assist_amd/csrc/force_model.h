@@ -115,53 +115,61 @@ struct ChebOut {
     D vx, vy, vz;  // km/day (only when requested)
 };
 
-template <typename D>
-AHD ChebOut<D> cheb_eval(const DevSlot& S, D t, bool want_vel) {
+struct alignas(16) Pair { double x, y; };
+
+// The device evaluates all kCoefPad terms of the zero-padded record with every load issued
+// up front; the operation-count instantiation (D != double) counts the file's P terms.
+template <typename D, bool Vel>
+AHD ChebOut<D> cheb_eval(const DevSlot& S, D t) {
     const D fi = (D(S.c_init) + t) * D(S.inv_intlen);
     int idx = (int)value_of(fi);
     idx = idx < 0 ? 0 : (idx >= S.nrec ? S.nrec - 1 : idx);
-    const double* rec = S.rec + (size_t)idx * S.R;
-    const D z = (D(rec[0]) + t) * D(rec[1]);
-    const int P = S.P;
-    const double* cx = rec + 2;
-    const double* cy = cx + P;
-    const double* cz = cy + P;
+    const double* rec = S.rec + (size_t)idx * kRecWords;
+    double c[kRecWords];
+#pragma unroll
+    for (int k = 0; k < kRecWords; k += 2) {
+        const Pair v = *reinterpret_cast<const Pair*>(rec + k);
+        c[k] = v.x;
+        c[k + 1] = v.y;
+    }
+    const int P = std::is_same<D, double>::value ? kCoefPad : S.P;
+    const D z = (D(c[0]) + t) * D(c[1]);
+    const double* cx = c + 2;
+    const double* cy = cx + kCoefPad;
+    const double* cz = cy + kCoefPad;
     const D z2 = D(2.0) * z;
     D T0 = D(1.0), T1 = z;
     D x = fma(D(cx[1]), z, D(cx[0]));
     D y = fma(D(cy[1]), z, D(cy[0]));
     D w = fma(D(cz[1]), z, D(cz[0]));
     ChebOut<D> o;
-    if (!want_vel) {
-        for (int p = 2; p < P; p++) {
+    D S0 = D(0.0), S1 = D(1.0);
+    D vx = D(cx[1]), vy = D(cy[1]), vz = D(cz[1]);
+#pragma unroll
+    for (int p = 2; p < kCoefPad; p++) {
+        if (p < P) {
             const D T2 = fma(z2, T1, -T0);
             x = fma(D(cx[p]), T2, x);
             y = fma(D(cy[p]), T2, y);
             w = fma(D(cz[p]), T2, w);
+            if (Vel) {
+                const D S2 = fma(z2, S1, fma(D(2.0), T1, -S0));
+                vx = fma(D(cx[p]), S2, vx);
+                vy = fma(D(cy[p]), S2, vy);
+                vz = fma(D(cz[p]), S2, vz);
+                S0 = S1;
+                S1 = S2;
+            }
             T0 = T1;
             T1 = T2;
         }
-        o.vx = o.vy = o.vz = D(0.0);
+    }
+    if (Vel) {
+        o.vx = vx * D(c[1]);
+        o.vy = vy * D(c[1]);
+        o.vz = vz * D(c[1]);
     } else {
-        D S0 = D(0.0), S1 = D(1.0);
-        D vx = D(cx[1]), vy = D(cy[1]), vz = D(cz[1]);
-        for (int p = 2; p < P; p++) {
-            const D T2 = fma(z2, T1, -T0);
-            const D S2 = fma(z2, S1, fma(D(2.0), T1, -S0));
-            x = fma(D(cx[p]), T2, x);
-            y = fma(D(cy[p]), T2, y);
-            w = fma(D(cz[p]), T2, w);
-            vx = fma(D(cx[p]), S2, vx);
-            vy = fma(D(cy[p]), S2, vy);
-            vz = fma(D(cz[p]), S2, vz);
-            T0 = T1;
-            T1 = T2;
-            S0 = S1;
-            S1 = S2;
-        }
-        o.vx = vx * D(rec[1]);
-        o.vy = vy * D(rec[1]);
-        o.vz = vz * D(rec[1]);
+        o.vx = o.vy = o.vz = D(0.0);
     }
     o.x = x;
     o.y = y;
@@ -495,6 +503,12 @@ struct EvalResult {
     int status;
 };
 
+// Distance [au] inside which the mixed form evaluates an asteroid's point mass in FP64.  The
+// FP32 round-off of the term (~6e-8 of it) is amplified ~5e3 times in IAS15's b6 error
+// estimate; beyond 0.2 au even the largest asteroid's term is < 1e-6 of the solar
+// acceleration, so the amplified noise stays ~1e-10, below eps = 1e-9.
+constexpr double kFp32MinDist = 0.2;
+
 template <typename D, bool Mixed>
 struct TermScalar { using type = typename std::conditional<Mixed, typename SingleOf<D>::type, D>::type; };
 
@@ -512,22 +526,25 @@ AHD void force_eval(const DevEphem& E, const ForceOpts& O, int lane, D t, const 
     const bool eih = (forces & F_GR_EIH) && O.gr_eih_sources > 0;
     const bool want_sun_vel = eih || ((forces & F_NONGRAV) && ng) || (forces & F_GR_SIMPLE);
 
-    // 1. Chebyshev evaluations of the slots owned by this lane
+    // 1. Chebyshev evaluations of the slots owned by this lane; round 0 (which holds the Sun,
+    //    slot 0, on lane 0) also evaluates velocities when the Sun's velocity is needed
     D px[NR][3];
+    D sunv_l[3] = {D(0.0), D(0.0), D(0.0)};
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         const int s = r * G + lane;
         if (s < kNumSlots) {
-            const ChebOut<D> c = cheb_eval<D>(E.slot[s], t, false);
+            ChebOut<D> c;
+            if (r == 0 && want_sun_vel) {
+                c = cheb_eval<D, true>(E.slot[s], t);
+                sunv_l[0] = c.vx; sunv_l[1] = c.vy; sunv_l[2] = c.vz;
+            } else {
+                c = cheb_eval<D, false>(E.slot[s], t);
+            }
             px[r][0] = c.x; px[r][1] = c.y; px[r][2] = c.z;
         } else {
             px[r][0] = px[r][1] = px[r][2] = D(0.0);
         }
-    }
-    D sunv_l[3] = {D(0.0), D(0.0), D(0.0)};
-    if (want_sun_vel && lane == 0) {
-        const ChebOut<D> c = cheb_eval<D>(E.slot[0], t, true);
-        sunv_l[0] = c.vx; sunv_l[1] = c.vy; sunv_l[2] = c.vz;
     }
     // 2. exchange Sun (au) and EMB (km); compose Earth, Moon and asteroids
     D sun[3], sunv[3], emb[3];
@@ -570,7 +587,10 @@ AHD void force_eval(const DevEphem& E, const ForceOpts& O, int lane, D t, const 
         const D GM = D(E.GM[b]);
         const bool allowed = (b == 0) ? (forces & F_SUN) : (b < kNumPlanets ? (forces & F_PLANETS) : (forces & F_ASTEROIDS));
         const D dx = xr[0] - px[r][0], dy = xr[1] - px[r][1], dz = xr[2] - px[r][2];
-        if (Mixed && b >= kNumPlanets) {
+        // Mixed form: an asteroid's point mass runs in FP32 unless the particle is within
+        // kFp32MinDist of it, where the FP32 round-off of the term would reach IAS15's
+        // error estimate (see kFp32MinDist).
+        if (Mixed && b >= kNumPlanets && value_of(dx * dx + dy * dy + dz * dz) > kFp32MinDist * kFp32MinDist) {
             const TS fx = TS(dx), fy = TS(dy), fz = TS(dz);
             const TS r2 = fx * fx + fy * fy + fz * fz;
             const TS rr = sqrt(r2);

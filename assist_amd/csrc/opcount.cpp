@@ -65,16 +65,15 @@ using namespace assist_gpu;
 static DevEphem fake_ephem(std::vector<double>& buf) {
     DevEphem E;
     memset(&E, 0, sizeof E);
-    buf.assign(64 * kNumSlots, 0.5);
+    buf.assign(kRecWords * kNumSlots, 0.5);
     for (int s = 0; s < kNumSlots; s++) {
         const int P = (s == kSlotEarth || s == kSlotMoon) ? 13 : (s < kFirstAsteroidSlot ? 14 : 11);
         DevSlot& S = E.slot[s];
-        S.rec = buf.data() + 64 * s;
+        S.rec = buf.data() + kRecWords * s;
         S.c_init = 0.0;
         S.inv_intlen = 0.0;
         S.nrec = 1;
         S.P = P;
-        S.R = ((2 + 3 * P) + 1) & ~1;
     }
     for (int b = 0; b < kNumBodies; b++) E.GM[b] = 1e-10 * (b + 1);
     E.inv_au = 1.0 / 149597870.7;

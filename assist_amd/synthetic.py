@@ -188,6 +188,7 @@ class SyntheticModel:
     """Analytic body positions of the synthetic solar system (seeded)."""
 
     seed: int = 0
+    planet_mass_scale: float = 1.0  # 0 gives massless planets: the Sun rests at the barycentre
 
     def __post_init__(self):
         rng = np.random.default_rng(self.seed)
@@ -195,7 +196,7 @@ class SyntheticModel:
         self.emrat = CONSTANTS["EMRAT"]
         self.planets = {}
         for naif, (name, key, a, e, i, node, varpi, L) in PLANET_ELEMENTS.items():
-            gm = GM_PLANETS[key]
+            gm = GM_PLANETS[key] * self.planet_mass_scale
             argp = math.radians(varpi - node)
             M0 = math.radians(L - varpi)
             self.planets[naif] = dict(gm=gm, a=a, e=e, i=math.radians(i), node=math.radians(node), argp=argp, M0=M0)
@@ -401,31 +402,32 @@ def _segment(model, code, center, rec):
     return SpkSegment(code, center, t0 * DAY_S, rec_days * DAY_S, mids, radii, coeffs)
 
 
-def comment_lines(model):
+def comment_lines(model, constants=None):
     lines = [
         "ASSIST-AMD synthetic ephemeris (seed %d): Kepler orbits fitted with Chebyshev records." % model.seed,
         "Not a physical ephemeris; for tests and benchmarks only.",
         "",
         "Initial conditions and constants used for integration:",
     ]
-    for k, v in CONSTANTS.items():
+    for k, v in dict(CONSTANTS, **(constants or {})).items():
         lines.append("%-8s %s" % (k, _fmt_d(v)))
     for k, v in GM_PLANETS.items():
-        lines.append("%-8s %s" % (k, _fmt_d(v)))
+        lines.append("%-8s %s" % (k, _fmt_d(v if k == "GMS" else v * model.planet_mass_scale)))
     for a in model.asteroids:
         lines.append("MA%04d   %s" % (a["num"], _fmt_d(a["gm"])))
     return lines
 
 
-def write_synthetic_ephemeris(planets_path, asteroids_path, seed=0):
-    """Write the synthetic planets and asteroid SPK files; returns the model."""
-    model = SyntheticModel(seed)
+def write_synthetic_ephemeris(planets_path, asteroids_path, seed=0, planet_mass_scale=1.0, constants=None):
+    """Write the synthetic planets and asteroid SPK files; returns the model.  ``constants``
+    overrides entries of the comment-area constants (tests isolate J2/J3/J4 this way)."""
+    model = SyntheticModel(seed, planet_mass_scale)
     planet_segs = [_segment(model, 10, 0, PLANET_RECORD)]
     for naif in (1, 2, 3, 4, 5, 6, 7, 8, 9):
         planet_segs.append(_segment(model, naif, 0, PLANET_RECORD))
     planet_segs.append(_segment(model, 399, 3, MOON_RECORD))
     planet_segs.append(_segment(model, 301, 3, MOON_RECORD))
-    lines = comment_lines(model)
+    lines = comment_lines(model, constants)
     write_spk(planets_path, planet_segs, lines, "ASSIST-AMD SYNTHETIC PLANETS")
     ast_segs = [_segment(model, 2000000 + a["num"], 10, ASTEROID_RECORD) for a in model.asteroids]
     write_spk(asteroids_path, ast_segs, lines[:2], "ASSIST-AMD SYNTHETIC ASTEROIDS")

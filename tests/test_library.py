@@ -1,0 +1,185 @@
+"""CPU tests of the C-ABI boundary and the Python drop-in (no GPU calls):
+
+* libassist.so loads and exports every function include/assist.h declares;
+* format detection on the byte samples of the reference's own test
+  (unit_tests/spk_detection/problem.c:14-46, fixtures in tests/golden);
+* host-side ephemeris queries agree with the oracle bit for bit;
+* the Python wrapper's names, layouts and errors mirror the reference wrapper
+  (assist/ephem.py, assist/extras.py, assist/test/test_forces.py:10-22,
+  assist/test/test_basic.py:34-41);
+* GPU entry points fail loudly (no CPU fallback) when no GPU is visible.
+"""
+import ctypes as C
+import os
+import re
+import struct
+import tempfile
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _declared_functions():
+    """Names of the functions include/assist.h declares (function-pointer members excluded)."""
+    txt = open(os.path.join(ROOT, "include", "assist.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    txt = re.sub(r"//[^\n]*", "", txt)
+    txt = re.sub(r"^\s*#[^\n]*", "", txt, flags=re.M)
+    names = set()
+    for chunk in txt.split(";"):
+        chunk = chunk.split("{")[-1].split("}")[-1]
+        if "(" not in chunk or "(*" in chunk.replace(" ", "") or "typedef" in chunk:
+            continue
+        m = re.search(r"(\w+)\s*\(", chunk)
+        if m:
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    import assist_amd
+    lib = assist_amd.clibassist
+    names = _declared_functions()
+    assert len(names) >= 38
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    for data in ("assist_version_str", "assist_build_str", "assist_githash_str", "assist_error_messages",
+                 "assist_error_messages_N"):
+        assert C.c_void_p.in_dll(lib, data) is not None
+
+
+def test_version_and_error_messages():
+    import assist_amd
+    assert assist_amd.__version__ == "1.2.0"
+    assert assist_amd.assist_error_messages(5).startswith("The requested time is outside the coverage")
+    with pytest.raises(RuntimeError):
+        assist_amd.assist_error_messages(6)
+
+
+def _detect(path):
+    import assist_amd
+    fd = os.open(path, os.O_RDONLY)
+    try:
+        return assist_amd.clibassist.assist_detect_ephemeris_file_format(fd)
+    finally:
+        os.close(fd)
+
+
+def test_format_detection_on_reference_samples(ephem_paths):
+    assert _detect(os.path.join(GOLDEN, "detect_valid_bsp_sample.bin")) == 0      # FILE_FORMAT_VALID_BSP
+    assert _detect(os.path.join(GOLDEN, "detect_ascii_bin_sample.bin")) == 2      # FILE_FORMAT_ASCII_BIN
+    assert _detect(os.path.join(GOLDEN, "detect_ascii_source_sample.txt")) == 3   # FILE_FORMAT_UNKNOWN
+    assert _detect(ephem_paths[0]) == 0
+
+
+def test_discovery_and_missing_files():
+    import assist_amd
+    buf = C.create_string_buffer(1024)
+    with tempfile.TemporaryDirectory() as d:
+        assert assist_amd.clibassist.assist_discover_planets_path(buf, 1024, d.encode()) == 0
+        os.makedirs(os.path.join(d, "data"))
+        open(os.path.join(d, "data", "de440.bsp"), "wb").close()
+        assert assist_amd.clibassist.assist_discover_planets_path(buf, 1024, d.encode()) == 1
+        assert buf.value.decode().endswith("/data/de440.bsp")
+    with pytest.raises(RuntimeError):
+        assist_amd.Ephem("/nonexistent.bsp", None)
+
+
+def test_ephem_dropin(ephem, oracle):
+    import assist_amd
+    assert ephem.jd_ref == 2451545.0
+    assert ephem.time_bounds() == (-10000.0, 10000.0)
+    for body, t in ((0, 0.0), (1, 100.0), (3, -2500.5), (4, 77.7), (20, 200.0)):
+        p = ephem.get_particle(body, t)
+        f, o = oracle.all_ephem(body, t)
+        assert f == 0
+        assert (p.m, p.x, p.y, p.z) == (o[0], o[1], o[2], o[3])
+        if body < 11:
+            assert (p.vx, p.vy, p.vz, p.ax, p.ay, p.az) == tuple(o[4:10])
+    assert ephem.get_particle("Sun", 0).x == ephem.get_particle(0, 0).x
+    assert ephem.get_particle("vesta", 10).x == ephem.get_particle(26, 10).x
+    with pytest.raises(ValueError):
+        ephem.get_particle("Planet 9", 0)
+    with pytest.raises(RuntimeError):
+        ephem.get_particle(0, 1e6)
+    assert len(assist_amd.ASSIST_BODY_IDS) == 27
+
+
+def test_extras_dropin(ephem):
+    import assist_amd
+    sim = assist_amd.Simulation()
+    ex = assist_amd.Extras(sim, ephem)
+    assert ex.forces == ["SUN", "PLANETS", "ASTEROIDS", "NON_GRAVITATIONAL", "EARTH_HARMONICS", "SUN_HARMONICS", "GR_EIH"]
+    assert ex.gr_eih_sources == 1
+    assert (ex.alpha, ex.nk, ex.nm, ex.nn, ex.r0) == (1.0, 0.0, 2.0, 5.093, 1.0)
+    with pytest.raises(AttributeError):
+        ex.forces = "no array"
+    with pytest.raises(AttributeError):
+        ex.forces = ["Magic"]
+    with pytest.raises(AttributeError):
+        ex.forces = [1, 2, 3]
+    f = ex.forces
+    f.remove("GR_EIH")
+    ex.forces = f
+    assert "GR_EIH" not in ex.forces and len(ex.forces) == 6
+    ex.forces = ["sun"]  # accepted (upper-cased check) but, as in the reference, matches nothing
+    assert ex.forces == []
+    with pytest.raises(AttributeError):
+        _ = ex.particle_params
+    ex.particle_params = np.array([1e-12, 0.0, 0.0])
+    assert sim.ri_ias15.adaptive_mode == 1
+    ex.detach(sim)
+
+
+def test_struct_layouts_match_reference_offsets():
+    """Field offsets of the drop-in structs equal the reference layouts (LP64)."""
+    import assist_amd
+    from assist_amd.extras import Extras
+    E = assist_amd.Ephem
+    assert E.jd_ref.offset == 0 and E.spk_planets.offset == 8 and E.planets_source.offset == 32
+    assert E.planets_calc.offset == 40 and E.spk_target_index.offset == 48 and E.spk_emb_index.offset == 92
+    assert E.AU.offset == 96 and E.over_c_squared.offset == 96 + 13 * 8
+    assert Extras.steps_done.offset == 56 and Extras._forces.offset == 60 and Extras.gr_eih_sources.offset == 64
+    assert Extras.alpha.offset == 72 and Extras.r0.offset == 104
+
+
+def test_gpu_entry_points_fail_without_gpu(ephem):
+    from assist_amd import batch
+    if batch.gpu_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(batch.BatchError):
+        batch.forces(ephem, 0.0, np.zeros((1, 1, 6)))
+    with pytest.raises(batch.BatchError):
+        batch.integrate(ephem, 0.0, 1.0, np.zeros((1, 1, 6)))
+
+
+def test_synthetic_file_satisfies_reader_assumptions(ephem_paths):
+    for path in ephem_paths:
+        buf = open(path, "rb").read()
+        assert buf[:8] == b"DAF/SPK "
+        nd, ni = struct.unpack_from("<ii", buf, 8)
+        assert (nd, ni) == (2, 6)
+        fward = struct.unpack_from("<i", buf, 76)[0]
+        off = (fward - 1) * 1024
+        nxt, prev, nsum = struct.unpack_from("<ddd", buf, off)
+        assert nxt == 0.0 and prev == 0.0 and buf[off + 8] == 0
+        assert len(buf) % 1024 == 0
+
+
+def test_elements_round_trip(model):
+    from assist_amd import synthetic
+    rng = np.random.default_rng(1)
+    a = rng.uniform(0.8, 3.5, 100)
+    e = rng.uniform(0, 0.6, 100)
+    inc = rng.uniform(0.05, 0.6, 100)
+    node, argp, M = (rng.uniform(0, 2 * np.pi, 100) for _ in range(3))
+    r, v = synthetic.kepler_state(model.gms, a, e, inc, node, argp, M)
+    a2, e2, i2, n2, w2, M2 = synthetic.state_to_elements(model.gms, r, v)
+    np.testing.assert_allclose(a2, a, rtol=1e-12)
+    np.testing.assert_allclose(e2, e, atol=1e-12)
+    np.testing.assert_allclose(i2, inc, atol=1e-12)
+    d = lambda x, y: np.abs((x - y + np.pi) % (2 * np.pi) - np.pi)
+    assert d(n2, node).max() < 1e-10 and d(w2 + M2, argp + M).max() < 1e-9
