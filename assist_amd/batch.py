@@ -27,7 +27,8 @@ PRECISION_MIXED = 1
 class BatchConfig(C.Structure):
     _fields_ = [("epsilon", C.c_double), ("min_dt", C.c_double), ("dt_initial", C.c_double),
                 ("max_steps", C.c_longlong), ("precision", C.c_int), ("lanes_per_system", C.c_int),
-                ("block_size", C.c_int), ("blocks_per_cu", C.c_int)]
+                ("block_size", C.c_int), ("blocks_per_cu", C.c_int), ("waves_per_simd", C.c_int),
+                ("reserved_", C.c_int)]
 
     @classmethod
     def default(cls, **kw):
@@ -41,7 +42,7 @@ class BatchConfig(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("steps", C.c_longlong), ("rejected", C.c_longlong), ("evals", C.c_longlong),
                 ("kernel_ms", C.c_double), ("lanes_per_system", C.c_int), ("block_size", C.c_int),
-                ("grid_blocks", C.c_int)]
+                ("grid_blocks", C.c_int), ("waves_per_simd", C.c_int)]
 
 
 clibassist.assist_batch_error_string.restype = C.c_char_p

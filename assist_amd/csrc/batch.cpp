@@ -153,6 +153,10 @@ int get_device_ephem(const assist_ephem* e, const DevEphem** out, DevEphem* host
     const double d2r = M_PI / 180.;
     H.e_cosa = cos(0.0 * d2r); H.e_sina = sin(0.0 * d2r); H.e_cosd = cos(90.0 * d2r); H.e_sind = sin(90.0 * d2r);
     H.s_cosa = cos(286.13 * d2r); H.s_sina = sin(286.13 * d2r); H.s_cosd = cos(63.87 * d2r); H.s_sind = sin(63.87 * d2r);
+    H.k_J2 = 3.0 * H.J2E * H.Re_eq * H.Re_eq / 2.0;
+    H.k_J3 = 5.0 * H.J3E * H.Re_eq * H.Re_eq * H.Re_eq / 2.0;
+    H.k_J4 = 5.0 * H.J4E * H.Re_eq * H.Re_eq * H.Re_eq * H.Re_eq / 8.0;
+    H.k_sJ2 = 3.0 * H.J2SUN * H.Rs_eq * H.Rs_eq / 2.0;
     c = new DeviceEphemCache();
     c->device = dev;
     c->jd_ref = e->jd_ref;
@@ -399,12 +403,14 @@ int assist_batch_integrate_device(const struct assist_ephem* ephem, const struct
     const int G = lanes_for(nv, cfg.lanes_per_system);
     if (G < 0) return ASSIST_BATCH_ERROR_ARG;
     const bool mixed = cfg.precision == ASSIST_PRECISION_MIXED;
-    const int threads = cfg.block_size > 0 ? cfg.block_size : 256;
-    if (threads % 64 != 0 || threads > 256) return ASSIST_BATCH_ERROR_ARG;
+    const int threads = cfg.block_size > 0 ? cfg.block_size : kIas15Block;
+    if (threads % 64 != 0 || threads > kIas15Block) return ASSIST_BATCH_ERROR_ARG;
+    const int waves = cfg.waves_per_simd > 0 ? cfg.waves_per_simd : 1;
+    if (waves != 1 && waves != 2) return ASSIST_BATCH_ERROR_ARG;
     int dev = 0, ncu = 0, per_cu = 0;
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice") ||
         !hip_ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute") ||
-        !hip_ok(ias15_occupancy(G, mixed, threads, &per_cu), "occupancy"))
+        !hip_ok(ias15_occupancy(G, mixed, waves, nv, threads, &per_cu), "occupancy"))
         return ASSIST_BATCH_ERROR_DEVICE;
     if (per_cu < 1) per_cu = 1;
     if (cfg.blocks_per_cu > 0 && cfg.blocks_per_cu < per_cu) per_cu = cfg.blocks_per_cu;
@@ -432,7 +438,7 @@ int assist_batch_integrate_device(const struct assist_ephem* ephem, const struct
     hipStream_t s = (hipStream_t)stream;
     if (!hip_ok(launch_set_counter(g_ws.queue, total_groups, s), "launch(set_counter)") ||
         !hip_ok(hipEventRecord(g_ws.ev0, s), "hipEventRecord") ||
-        !hip_ok(launch_ias15(a, G, mixed, blocks, threads, s), "launch(ias15)") ||
+        !hip_ok(launch_ias15(a, G, mixed, waves, blocks, threads, s), "launch(ias15)") ||
         !hip_ok(hipEventRecord(g_ws.ev1, s), "hipEventRecord") ||
         !hip_ok(hipEventSynchronize(g_ws.ev1), "ias15"))
         return ASSIST_BATCH_ERROR_DEVICE;
@@ -443,6 +449,7 @@ int assist_batch_integrate_device(const struct assist_ephem* ephem, const struct
         stats->lanes_per_system = G;
         stats->block_size = threads;
         stats->grid_blocks = blocks;
+        stats->waves_per_simd = waves;
     }
     return ASSIST_SUCCESS;
 }

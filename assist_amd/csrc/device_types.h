@@ -63,6 +63,8 @@ struct DevEphem {
     // cos/sin computed on the host (reference src/forces.c:484-490,670-676)
     double e_cosa, e_sina, e_cosd, e_sind;
     double s_cosa, s_sina, s_cosd, s_sind;
+    // zonal-harmonic prefactors: 3 J2E Re^2 / 2, 5 J3E Re^3 / 2, 5 J4E Re^4 / 8, 3 J2SUN Rs^2 / 2
+    double k_J2, k_J3, k_J4, k_sJ2;
 };
 
 struct ForceOpts {

@@ -51,20 +51,46 @@ enum : int {
 };
 
 // ---- lane-group collectives ----------------------------------------------------------
-// Butterfly collectives over the G lanes of a group (G = 1: identity).
+// Butterfly collectives over the G lanes of a group (G = 1: identity), on DPP lane
+// permutations within a row: quad_perm [1,0,3,2] pairs lanes i and i^1, quad_perm [2,3,0,1]
+// pairs i and i^2, row_half_mirror pairs lane i of an 8-lane half-row with lane 7-i (after the
+// first two steps every lane of a quad holds the quad total, so this adds the other quad's).
+// Floating-point addition is commutative, so every lane ends with bit-identical totals.
+#if defined(__HIP_DEVICE_COMPILE__)
+enum : int { kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141 };
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i<CTRL>((int)(b & 0xffffffffLL));
+    const int hi = dpp_i<CTRL>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+#endif
+
 template <int G, typename T>
 AHD T gsum(T v) {
+    static_assert(G == 1 || G == 2 || G == 4 || G == 8, "group size");
 #if defined(__HIP_DEVICE_COMPILE__)
-#pragma unroll
-    for (int m = 1; m < G; m <<= 1) v += __shfl_xor(v, m, G);
+    if constexpr (G >= 2) v += dpp<kDppXor1>(v);
+    if constexpr (G >= 4) v += dpp<kDppXor2>(v);
+    if constexpr (G >= 8) v += dpp<kDppHalfMirror>(v);
 #endif
     return v;
 }
 template <int G>
 AHD double gmax(double v) {
 #if defined(__HIP_DEVICE_COMPILE__)
-#pragma unroll
-    for (int m = 1; m < G; m <<= 1) v = fmax(v, __shfl_xor(v, m, G));
+    if constexpr (G >= 2) v = fmax(v, dpp<kDppXor1>(v));
+    if constexpr (G >= 4) v = fmax(v, dpp<kDppXor2>(v));
+    if constexpr (G >= 8) v = fmax(v, dpp<kDppHalfMirror>(v));
 #endif
     return v;
 }
@@ -87,8 +113,9 @@ AHD int gbcast_i(int v, int src) {
 template <int G>
 AHD int gor(int v) {
 #if defined(__HIP_DEVICE_COMPILE__)
-#pragma unroll
-    for (int m = 1; m < G; m <<= 1) v |= __shfl_xor(v, m, G);
+    if constexpr (G >= 2) v |= dpp_i<kDppXor1>(v);
+    if constexpr (G >= 4) v |= dpp_i<kDppXor2>(v);
+    if constexpr (G >= 8) v |= dpp_i<kDppHalfMirror>(v);
 #endif
     return v;
 }
@@ -98,6 +125,23 @@ template <typename D>
 struct SingleOf { using type = float; };
 
 AHD double value_of(double x) { return x; }
+
+// 1/sqrt(x): the hardware reciprocal square root with one refinement step on the GPU
+// (within a few ulp), replacing a square root and a division.
+AHD double inv_sqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return rsqrt(x);
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
+AHD float inv_sqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return rsqrtf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
 AHD float value_of(float x) { return x; }
 
 AHD bool is_normal(double x) {
@@ -223,72 +267,84 @@ AHD void add_rotated_sym(T J[3][3], const Rot<T>& R, T sxx, T syy, T szz, T sxy,
 // ---- single-particle terms ---------------------------------------------------------------
 
 // Earth J2/J3/J4 (reference src/forces.c:435-642).  J2 in D; J3/J4 and the partials in TS.
+// The Earth's pole is fixed at RA 0, Dec 90 deg (reference src/forces.c:484-490): the rotation
+// has cos RA = 1, sin RA = 0, sin Dec = 1 exactly and keeps cos Dec = cos(pi/2) = 6.1e-17,
+// whose residue the reference carries, so it is written out for that form (rows (0, 1, 0),
+// (-1, 0, c), (c, 0, 1) with c = cos Dec).  Powers of 1/r replace the reference's divisions.
 template <typename D, typename TS>
 AHD void term_earth_J2J4(const DevEphem& E, const D xr[3], const D xe[3], D GM, bool need_jac, D acc[3],
                          Jac<TS>& Jt) {
-    const D cosa = D(E.e_cosa), sina = D(E.e_sina), cosd = D(E.e_cosd), sind = D(E.e_sind);
-    D dx = xr[0] - xe[0], dy = xr[1] - xe[1], dz = xr[2] - xe[2];
-    const D r2 = dx * dx + dy * dy + dz * dz;
-    const D r = sqrt(r2);
-    const D dxp = -dx * sina + dy * cosa;
-    const D dyp = -dx * cosa * sind - dy * sina * sind + dz * cosd;
-    const D dzp = dx * cosa * cosd + dy * sina * cosd + dz * sind;
-    dx = dxp; dy = dyp; dz = dzp;
-    const D Re = D(E.Re_eq);
-    const D costheta2 = dz * dz / r2;
-    const D J2p = D(3.) * D(E.J2E) * Re * Re / r2 / r2 / r / D(2.);
+    const D c = D(E.e_cosd);
+    const D ex = xr[0] - xe[0], ey = xr[1] - xe[1], ez = xr[2] - xe[2];
+    const D r2 = ex * ex + ey * ey + ez * ez;
+    const D ir = inv_sqrt(r2);
+    const D ir2 = ir * ir;
+    const D dx = ey, dy = -ex + ez * c, dz = ex * c + ez;
+    const D costheta2 = dz * dz * ir2;
+    const D J2p = D(E.k_J2) * (ir2 * ir2 * ir);  // 3 J2 Re^2 / (2 r^5)
     const D J2f = D(5.) * costheta2 - D(1.);
     D rx = GM * J2p * J2f * dx;
     D ry = GM * J2p * J2f * dy;
     D rz = GM * J2p * (J2f - D(2.)) * dz;
-    const TS ct2 = TS(costheta2), tdx = TS(dx), tdy = TS(dy), tdz = TS(dz), tr2 = TS(r2), tr = TS(r), tGM = TS(GM);
-    const TS tRe = TS(Re);
-    const TS J3p = TS(5.) * TS(E.J3E) * tRe * tRe * tRe / tr2 / tr2 / tr / TS(2.);
+    const TS ct2 = TS(costheta2), tdx = TS(dx), tdy = TS(dy), tdz = TS(dz), tGM = TS(GM);
+    const TS tir = TS(ir), tir2 = TS(ir2);
+    const TS tir5 = tir2 * tir2 * tir;
+    const TS J3p = TS(E.k_J3) * tir5;            // 5 J3 Re^3 / (2 r^5)
     const TS J3f = TS(3.) - TS(7.) * ct2;
-    const TS J4p = TS(5.) * TS(E.J4E) * tRe * tRe * tRe * tRe / tr2 / tr2 / tr2 / tr / TS(8.);
+    const TS J4p = TS(E.k_J4) * (tir5 * tir2);   // 5 J4 Re^4 / (8 r^7)
     const TS J4f = TS(63.) * ct2 * ct2 - TS(42.) * ct2 + TS(3.);
-    const TS j3x = -tGM * J3p * (TS(1.) / tr2) * J3f * tdx * tdz;
-    const TS j3y = -tGM * J3p * (TS(1.) / tr2) * J3f * tdy * tdz;
+    const TS j3x = -tGM * J3p * tir2 * J3f * tdx * tdz;
+    const TS j3y = -tGM * J3p * tir2 * J3f * tdy * tdz;
     const TS j3z = -tGM * J3p * (TS(6.) * ct2 - TS(7.) * ct2 * ct2 - TS(0.6));
     const TS j4x = tGM * J4p * J4f * tdx;
     const TS j4y = tGM * J4p * J4f * tdy;
     const TS j4z = tGM * J4p * (J4f + TS(12.) - TS(28.) * ct2) * tdz;
     rx += D(j3x); ry += D(j3y); rz += D(j3z);
     rx += D(j4x); ry += D(j4y); rz += D(j4z);
-    acc[0] += -rx * sina - ry * cosa * sind + rz * cosa * cosd;
-    acc[1] += rx * cosa - ry * sina * sind + rz * sina * cosd;
-    acc[2] += +ry * cosd + rz * sind;
+    acc[0] += -ry + rz * c;
+    acc[1] += rx;
+    acc[2] += ry * c + rz;
     if (!need_jac) return;
     const TS J2pT = TS(J2p), J2fT = TS(J2f);
     const TS f2 = TS(7.) * ct2 - TS(1.);
     const TS f3 = TS(35.) * ct2 * ct2 - TS(30.) * ct2 + TS(3.);
-    TS sxx = tGM * J2pT * (J2fT - TS(5.) * f2 * tdx * tdx / tr2);
-    TS syy = tGM * J2pT * (J2fT - TS(5.) * f2 * tdy * tdy / tr2);
+    TS sxx = tGM * J2pT * (J2fT - TS(5.) * f2 * tdx * tdx * tir2);
+    TS syy = tGM * J2pT * (J2fT - TS(5.) * f2 * tdy * tdy * tir2);
     TS szz = tGM * J2pT * TS(-1.) * f3;
-    TS sxy = tGM * J2pT * TS(-5.) * f2 * tdx * tdy / tr2;
-    TS syz = tGM * J2pT * TS(-5.) * (f2 - TS(2.)) * tdy * tdz / tr2;
-    TS sxz = tGM * J2pT * TS(-5.) * (f2 - TS(2.)) * tdx * tdz / tr2;
-    const TS ct = tdz / tr;
-    const TS g2 = TS(21) * (TS(-3.) * ct2 + TS(1.)) / tr2;
-    const TS g3 = TS(3) * (TS(-21.) * ct2 * ct2 + TS(14.) * ct2 - TS(1.)) / tr2;
-    const TS g4 = (TS(-63.) * ct2 * ct2 + TS(70.) * ct2 - TS(15.)) * ct / tr;
-    sxx += tGM * J3p * ct * (g2 * tdx * tdx - J3f) / tr;
-    syy += tGM * J3p * ct * (g2 * tdy * tdy - J3f) / tr;
+    TS sxy = tGM * J2pT * TS(-5.) * f2 * tdx * tdy * tir2;
+    TS syz = tGM * J2pT * TS(-5.) * (f2 - TS(2.)) * tdy * tdz * tir2;
+    TS sxz = tGM * J2pT * TS(-5.) * (f2 - TS(2.)) * tdx * tdz * tir2;
+    const TS ct = tdz * tir;
+    const TS g2 = TS(21) * (TS(-3.) * ct2 + TS(1.)) * tir2;
+    const TS g3 = TS(3) * (TS(-21.) * ct2 * ct2 + TS(14.) * ct2 - TS(1.)) * tir2;
+    const TS g4 = (TS(-63.) * ct2 * ct2 + TS(70.) * ct2 - TS(15.)) * ct * tir;
+    sxx += tGM * J3p * ct * (g2 * tdx * tdx - J3f) * tir;
+    syy += tGM * J3p * ct * (g2 * tdy * tdy - J3f) * tir;
     szz += tGM * J3p * g4;
-    sxy += tGM * J3p * g2 * ct * tdx * tdy / tr;
+    sxy += tGM * J3p * g2 * ct * tdx * tdy * tir;
     syz += tGM * J3p * g3 * tdy;
     sxz += tGM * J3p * g3 * tdx;
     const TS h2 = TS(33.) * ct2 * ct2 - TS(18.) * ct2 + TS(1.);
     const TS h3 = TS(33.) * ct2 * ct2 - TS(30.) * ct2 + TS(5.);
     const TS h4 = TS(231.) * ct2 * ct2 * ct2 - TS(315.) * ct2 * ct2 + TS(105.) * ct2 - TS(5.);
-    sxx += tGM * J4p * (J4f - TS(21.) * h2 * tdx * tdx / tr2);
-    syy += tGM * J4p * (J4f - TS(21.) * h2 * tdy * tdy / tr2);
+    sxx += tGM * J4p * (J4f - TS(21.) * h2 * tdx * tdx * tir2);
+    syy += tGM * J4p * (J4f - TS(21.) * h2 * tdy * tdy * tir2);
     szz += tGM * J4p * TS(-3.) * h4;
-    sxy += tGM * J4p * TS(-21.) * h2 * tdx * tdy / tr2;
-    syz += tGM * J4p * TS(-21.) * h3 * tdy * tdz / tr2;
-    sxz += tGM * J4p * TS(-21.) * h3 * tdx * tdz / tr2;
-    Rot<TS> R(E.e_cosa, E.e_sina, E.e_cosd, E.e_sind);
-    add_rotated_sym(Jt.x, R, sxx, syy, szz, sxy, syz, sxz);
+    sxy += tGM * J4p * TS(-21.) * h2 * tdx * tdy * tir2;
+    syz += tGM * J4p * TS(-21.) * h3 * tdy * tdz * tir2;
+    sxz += tGM * J4p * TS(-21.) * h3 * tdx * tdz * tir2;
+    // J += R^T S R; R's columns are u0 = (0, -1, c), u1 = (1, 0, 0), u2 = (0, c, 1) and
+    // (R^T S R)_ij = u_i . (S u_j)
+    const TS tc = TS(c);
+    const TS w[3][3] = {{-sxy + tc * sxz, -syy + tc * syz, -syz + tc * szz},
+                        {sxx, sxy, sxz},
+                        {tc * sxy + sxz, tc * syy + syz, tc * syz + szz}};
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        Jt.x[0][j] += -w[j][1] + tc * w[j][2];
+        Jt.x[1][j] += w[j][0];
+        Jt.x[2][j] += tc * w[j][1] + w[j][2];
+    }
 }
 
 // Solar J2 (reference src/forces.c:644-772), in D; partials in TS.
@@ -297,13 +353,14 @@ AHD void term_solar_J2(const DevEphem& E, const D xr[3], const D xs[3], D GMs, b
     const D cosa = D(E.s_cosa), sina = D(E.s_sina), cosd = D(E.s_cosd), sind = D(E.s_sind);
     D dx = xr[0] - xs[0], dy = xr[1] - xs[1], dz = xr[2] - xs[2];
     const D r2 = dx * dx + dy * dy + dz * dz;
-    const D r = sqrt(r2);
+    const D ir = inv_sqrt(r2);
+    const D ir2 = ir * ir;
     const D dxp = -dx * sina + dy * cosa;
     const D dyp = -dx * cosa * sind - dy * sina * sind + dz * cosd;
     const D dzp = dx * cosa * cosd + dy * sina * cosd + dz * sind;
     dx = dxp; dy = dyp; dz = dzp;
-    const D costheta2 = dz * dz / r2;
-    const D pf = D(3.) * D(E.J2SUN) * D(E.Rs_eq) * D(E.Rs_eq) / r2 / r2 / r / D(2.);
+    const D costheta2 = dz * dz * ir2;
+    const D pf = D(E.k_sJ2) * (ir2 * ir2 * ir);  // 3 J2sun Rs^2 / (2 r^5)
     const D f1 = D(5.) * costheta2 - D(1.);
     const D rx = GMs * pf * f1 * dx;
     const D ry = GMs * pf * f1 * dy;
@@ -312,16 +369,16 @@ AHD void term_solar_J2(const DevEphem& E, const D xr[3], const D xs[3], D GMs, b
     acc[1] += rx * cosa - ry * sina * sind + rz * sina * cosd;
     acc[2] += +ry * cosd + rz * sind;
     if (!need_jac) return;
-    const TS t2 = TS(costheta2), tdx = TS(dx), tdy = TS(dy), tdz = TS(dz), tr2 = TS(r2);
+    const TS t2 = TS(costheta2), tdx = TS(dx), tdy = TS(dy), tdz = TS(dz), tir2 = TS(ir2);
     const TS gp = TS(GMs * pf), tf1 = TS(f1);
     const TS f2 = TS(7.) * t2 - TS(1.);
     const TS f3 = TS(35.) * t2 * t2 - TS(30.) * t2 + TS(3.);
-    const TS sxx = gp * (tf1 - TS(5.) * f2 * tdx * tdx / tr2);
-    const TS syy = gp * (tf1 - TS(5.) * f2 * tdy * tdy / tr2);
+    const TS sxx = gp * (tf1 - TS(5.) * f2 * tdx * tdx * tir2);
+    const TS syy = gp * (tf1 - TS(5.) * f2 * tdy * tdy * tir2);
     const TS szz = gp * TS(-1.) * f3;
-    const TS sxy = gp * TS(-5.) * f2 * tdx * tdy / tr2;
-    const TS syz = gp * TS(-5.) * (f2 - TS(2.)) * tdy * tdz / tr2;
-    const TS sxz = gp * TS(-5.) * (f2 - TS(2.)) * tdx * tdz / tr2;
+    const TS sxy = gp * TS(-5.) * f2 * tdx * tdy * tir2;
+    const TS syz = gp * TS(-5.) * (f2 - TS(2.)) * tdy * tdz * tir2;
+    const TS sxz = gp * TS(-5.) * (f2 - TS(2.)) * tdx * tdz * tir2;
     Rot<TS> R(E.s_cosa, E.s_sina, E.s_cosd, E.s_sind);
     add_rotated_sym(Jt.x, R, sxx, syy, szz, sxy, syz, sxz);
 }
@@ -392,8 +449,10 @@ AHD void term_eih_sun(const DevEphem& E, const D xr[3], const D vr[3], const D x
     const TS vxj = TS(vs[0]), vyj = TS(vs[1]), vzj = TS(vs[2]);
     const TS GMj = TS(GMs);
     const TS rij2 = dxij * dxij + dyij * dyij + dzij * dzij;
-    const TS _rij = sqrt(rij2);
-    const TS prefacij = GMj / (rij2 * _rij);
+    const TS ir = inv_sqrt(rij2);
+    const TS ir2 = ir * ir;
+    const TS ir3 = ir2 * ir;
+    const TS prefacij = GMj * ir3;
     const TS vi2 = vx * vx + vy * vy + vz * vz;
     const TS term2 = oc2 * vi2;
     const TS vj2 = vxj * vxj + vyj * vyj + vzj * vzj;
@@ -401,7 +460,7 @@ AHD void term_eih_sun(const DevEphem& E, const D xr[3], const D vr[3], const D x
     const TS vidotvj = vx * vxj + vy * vyj + vz * vzj;
     const TS term4 = TS(-4) * oc2 * vidotvj;
     const TS rijdotvj = dxij * vxj + dyij * vyj + dzij * vzj;
-    const TS term5 = TS(-1.5) * oc2 * (rijdotvj * rijdotvj) / (_rij * _rij);
+    const TS term5 = TS(-1.5) * oc2 * (rijdotvj * rijdotvj) * ir2;
     const TS fx = TS(4) * vx - TS(3) * vxj, fy = TS(4) * vy - TS(3) * vyj, fz = TS(4) * vz - TS(3) * vzj;
     const TS f = dxij * fx + dyij * fy + dzij * fz;
     const TS pf = prefacij * f;
@@ -410,25 +469,24 @@ AHD void term_eih_sun(const DevEphem& E, const D xr[3], const D vr[3], const D x
     const TS term1 = TS(S1) * (-oc2);
     const TS axj = TS(aj[0]), ayj = TS(aj[1]), azj = TS(aj[2]);
     const TS term6 = TS(-0.5) * oc2 * (dxij * axj + dyij * ayj + dzij * azj);
-    const TS t8 = GMj / _rij * TS(3.5);
+    const TS t8 = GMj * ir * TS(3.5);
     const TS factor = term0 + term1 + term2 + term3 + term4 + term5 + term6;
     const TS dij[3] = {dxij, dyij, dzij};
     const TS ajv[3] = {axj, ayj, azj};
 #pragma unroll
     for (int c = 0; c < 3; c++) acc[c] += D(-prefacij * dij[c] * factor + (pf * rel[c] * oc2 + t8 * ajv[c] * oc2));
     if (!need_jac) return;
-    const TS pv = GMj / (_rij * _rij * _rij);
-    const TS r5 = _rij * _rij * _rij * _rij * _rij;
-    const TS dpref[3] = {TS(-3.0) * GMj / r5 * dxij, TS(-3.0) * GMj / r5 * dyij, TS(-3.0) * GMj / r5 * dzij};
-    const TS t5f = TS(3.0) * oc2 * rijdotvj / _rij;
-    const TS r3 = _rij * _rij * _rij;
+    const TS pv = GMj * ir3;
+    const TS g5 = TS(-3.0) * GMj * (ir3 * ir2);
+    const TS dpref[3] = {g5 * dxij, g5 * dyij, g5 * dzij};
+    const TS t5f = TS(3.0) * oc2 * rijdotvj * ir;
     const TS vj[3] = {vxj, vyj, vzj};
     const TS vi[3] = {vx, vy, vz};
     const TS fv[3] = {fx, fy, fz};
     TS dfx[3], dfv[3];
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-        const TS dt5 = -t5f * (vj[q] / _rij - rijdotvj * dij[q] / r3);
+        const TS dt5 = -t5f * (vj[q] * ir - rijdotvj * dij[q] * ir3);
         const TS dt6 = TS(-0.5) * oc2 * ajv[q];
         dfx[q] = TS(dS0[q]) * (TS(-4) * oc2) + dt5 + dt6;
         dfv[q] = TS(2.0) * oc2 * vi[q] + TS(-4) * oc2 * vj[q];
@@ -441,7 +499,7 @@ AHD void term_eih_sun(const DevEphem& E, const D xr[3], const D vr[3], const D x
             const TS d7x = dpref[q] * f * rel[c] + pv * fv[q] * rel[c];
             TS d7v = pv * (TS(4) * dij[q]) * rel[c];
             if (q == c) d7v += pf7;
-            const TS d8 = -GMj * ajv[c] / r3 * dij[q] * TS(3.5);
+            const TS d8 = -GMj * ajv[c] * ir3 * dij[q] * TS(3.5);
             TS v = -dpref[q] * dij[c] * factor - pv * dij[c] * dfx[q];
             if (q == c) v -= pv * factor;
             J.x[c][q] += v + (d7x * oc2 + d8 * oc2);
@@ -593,12 +651,13 @@ AHD void force_eval(const DevEphem& E, const ForceOpts& O, int lane, D t, const 
         if (Mixed && b >= kNumPlanets && value_of(dx * dx + dy * dy + dz * dz) > kFp32MinDist * kFp32MinDist) {
             const TS fx = TS(dx), fy = TS(dy), fz = TS(dz);
             const TS r2 = fx * fx + fy * fy + fz * fz;
-            const TS rr = sqrt(r2);
-            const TS r3inv = TS(1) / (r2 * rr);
+            const TS ir = inv_sqrt(r2);
+            const TS ir2 = ir * ir;
+            const TS r3inv = ir2 * ir;
             const TS pref = TS(GM) * r3inv;
             if (allowed) { adt[0] -= pref * fx; adt[1] -= pref * fy; adt[2] -= pref * fz; }
             if (need_jac) {
-                const TS r5 = TS(3) * r3inv / r2;
+                const TS r5 = TS(3) * r3inv * ir2;
                 const TS g = TS(GM);
                 jd[0] += g * (fx * fx * r5 - r3inv); jd[1] += g * (fy * fy * r5 - r3inv); jd[2] += g * (fz * fz * r5 - r3inv);
                 jd[3] += g * (fx * fy * r5); jd[4] += g * (fy * fz * r5); jd[5] += g * (fx * fz * r5);
@@ -606,27 +665,28 @@ AHD void force_eval(const DevEphem& E, const ForceOpts& O, int lane, D t, const 
             continue;
         }
         const D r2 = dx * dx + dy * dy + dz * dz;
-        const D rr = sqrt(r2);
-        const D r3inv = D(1.0) / (r2 * rr);
+        const D ir = inv_sqrt(r2);
+        const D ir2 = ir * ir;
+        const D r3inv = ir2 * ir;
         const D pref = GM * r3inv;
         const D ax = -pref * dx, ay = -pref * dy, az = -pref * dz;
         if (allowed) { ad[0] += ax; ad[1] += ay; ad[2] += az; }
         if (need_jac) {
             const TS r3t = TS(r3inv);
-            const TS r5 = TS(D(3.0) * r3inv / r2);
+            const TS r5 = TS(D(3.0) * r3inv * ir2);
             const TS g = TS(GM), fx = TS(dx), fy = TS(dy), fz = TS(dz);
             jd[0] += g * (fx * fx * r5 - r3t); jd[1] += g * (fy * fy * r5 - r3t); jd[2] += g * (fz * fz * r5 - r3t);
             jd[3] += g * (fx * fy * r5); jd[4] += g * (fy * fz * r5); jd[5] += g * (fx * fz * r5);
         }
         if (eih && b < kNumPlanets) {
-            S0 += GM / rr;
+            S0 += GM * ir;
             dS0[0] += ax; dS0[1] += ay; dS0[2] += az;
             if (b != 0) {
                 const D sx = sun[0] - px[r][0], sy = sun[1] - px[r][1], sz = sun[2] - px[r][2];
                 const D q2 = sx * sx + sy * sy + sz * sz;
-                const D q = sqrt(q2);
-                S1 += GM / q;
-                const D f = GM / (q2 * q);
+                const D iq = inv_sqrt(q2);
+                S1 += GM * iq;
+                const D f = GM * (iq * iq * iq);
                 aj[0] -= f * sx; aj[1] -= f * sy; aj[2] -= f * sz;
             }
         }

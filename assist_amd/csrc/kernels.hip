@@ -29,6 +29,8 @@ __constant__ double cH[8] = IAS15_H_INIT;
 __constant__ double cRRI[28] = IAS15_RRINV_INIT;
 __constant__ double cC[21] = IAS15_C_INIT;
 __constant__ double cD[21] = IAS15_D_INIT;
+__constant__ double cPX[9 * 8] = IAS15_PX_INIT;
+__constant__ double cPV[9 * 7] = IAS15_PV_INIT;
 
 // ------------------------------------------------------------------------------------------
 // batched force evaluation
@@ -75,6 +77,30 @@ __global__ void __launch_bounds__(256) batch_forces_kernel(ForcesArgs a) {
 // ------------------------------------------------------------------------------------------
 // IAS15
 // ------------------------------------------------------------------------------------------
+// Kernel-argument pointer (constant address space, so loads through it are scalar loads; the
+// host compilation pass only type-checks the kernel body, with a plain pointer).
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(4))) const IntegrateArgs* KArgs;
+#else
+typedef const IntegrateArgs* KArgs;
+#endif
+__device__ __forceinline__ KArgs launder(KArgs p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+// A 7 x 3 array in a lane's LDS column: element (k, c) at p[3k + c].  A column holds b (words
+// 0-20) and g (21-41) and is padded to kLdsColWords doubles so that the 64 lanes of a wave,
+// reading the same element of their columns, spread over the LDS banks.
+struct LdsCol {
+    double* p;
+    __device__ double& operator[](int c) const { return p[c]; }
+};
+struct LdsArr {
+    double* p;
+    __device__ LdsCol operator[](int k) const { return LdsCol{p + 3 * k}; }
+};
+
 __device__ __forceinline__ void add_cs(double& p, double& cs, double in) {
     const double y = in - cs;
     const double t = p + y;
@@ -95,30 +121,41 @@ __device__ __forceinline__ void predict_e(const double br[7], double q, double e
     e[6] = q7 * br[6];
 }
 
-template <int G, bool Mixed>
-__global__ void __launch_bounds__(256) ias15_kernel(IntegrateArgs a) {
+template <int G, bool Mixed, int W>
+__global__ void __launch_bounds__(kIas15Block) __attribute__((amdgpu_waves_per_eu(W, W)))
+ias15_kernel(IntegrateArgs args) {
+    // The arguments are read through the kernel-argument segment pointer, which is
+    // re-laundered every loop trip: this keeps them out of SGPRs across the loop (where
+    // they would otherwise be spilled to VGPR lanes) at the price of a few scalar loads.
+    (void)args;
+    KArgs ka = (KArgs)__builtin_amdgcn_kernarg_segment_ptr();
     using TS = typename TermScalar<double, Mixed>::type;
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (G - 1);
     const int grp = tid / G;
-    const int nv = a.nvec;
+    const int nv = ka->nvec;
     const bool own = lane < nv;
-    const DevEphem& E = *a.E;
-    const IntegratorOpts io = a.io;
     // br[7][3], er[7][3] of the last accepted step, per lane, in the scratch buffer
-    double* scr = a.scratch + ((size_t)grp * G + lane) * 42;
+    double* scr = ka->scratch + ((size_t)grp * G + lane) * 42;
 
-    double x0[3], v0[3], csx[3], csv[3], a0[3], b[7][3], g[7][3];
+    double x0[3], v0[3], csx[3], csv[3], a0[3];
+    // b and g (7 x 3 each per vector) live in LDS, one column per owning lane, to relieve
+    // register pressure; lanes that own no vector share one spare column (their values are
+    // never used), which is what lets 7-vector systems on 8-lane groups fit two blocks per CU.
+    extern __shared__ double s_cols[];
+    const int cols_per_group = ias15_cols_per_group(G, nv);
+    const int col = own ? (int)(threadIdx.x / G) * cols_per_group + lane : (int)(blockDim.x / G) * cols_per_group;
+    const LdsArr b{s_cols + (size_t)col * kLdsColWords}, g{s_cols + (size_t)col * kLdsColWords + 21};
     double A[3], dA[3];
     double t = 0, t_end = 0, dt = 0, dts = 0, dt_last = 0, pc_err_last = 0;
     int phase = 0, n = 1, it = 0, last = 0, status = 0, attempts = 0;
     long long steps = 0, evals = 0, rejected = 0;
     bool ng = false;
     int sys = grp;
-    bool active = sys < a.nsys;
+    bool active = sys < ka->nsys;
 
     auto load = [&](int s) {
-        const double* st = a.states + ((size_t)s * nv + (own ? lane : 0)) * 6;
+        const double* st = ka->states + ((size_t)s * nv + (own ? lane : 0)) * 6;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             x0[c] = own ? st[c] : 0.0;
@@ -131,15 +168,15 @@ __global__ void __launch_bounds__(256) ias15_kernel(IntegrateArgs a) {
             for (int c = 0; c < 3; c++) b[k][c] = g[k][c] = 0.0;
         A[0] = A[1] = A[2] = dA[0] = dA[1] = dA[2] = 0.0;
         ng = false;
-        if (a.params) {
-            const double* pp = a.params + (size_t)s * nv * 3;
+        if (ka->params) {
+            const double* pp = ka->params + (size_t)s * nv * 3;
             A[0] = pp[0]; A[1] = pp[1]; A[2] = pp[2];
             ng = !(A[0] == 0.0 && A[1] == 0.0 && A[2] == 0.0);
             if (own) { dA[0] = pp[3 * lane]; dA[1] = pp[3 * lane + 1]; dA[2] = pp[3 * lane + 2]; }
         }
-        t = a.t0[s];
-        t_end = a.t1[s];
-        dt = io.dt_initial;
+        t = ka->t0[s];
+        t_end = ka->t1[s];
+        dt = ka->io.dt_initial;
         if (dt * (t_end - t) < 0) dt = -dt;
         dt_last = 0.0;
         phase = 0;
@@ -149,22 +186,22 @@ __global__ void __launch_bounds__(256) ias15_kernel(IntegrateArgs a) {
     };
     auto finish = [&]() {
         if (own) {
-            double* st = a.states + ((size_t)sys * nv + lane) * 6;
+            double* st = ka->states + ((size_t)sys * nv + lane) * 6;
 #pragma unroll
             for (int c = 0; c < 3; c++) { st[c] = x0[c]; st[3 + c] = v0[c]; }
         }
         if (lane == 0) {
-            if (a.status) a.status[sys] = status;
-            if (a.steps) a.steps[sys] = steps;
-            if (a.evals) a.evals[sys] = evals;
-            if (a.rejected) a.rejected[sys] = rejected;
-            if (a.t_out) a.t_out[sys] = t;
+            if (ka->status) ka->status[sys] = status;
+            if (ka->steps) ka->steps[sys] = steps;
+            if (ka->evals) ka->evals[sys] = evals;
+            if (ka->rejected) ka->rejected[sys] = rejected;
+            if (ka->t_out) ka->t_out[sys] = t;
         }
         int next = 0;
-        if (lane == 0) next = atomicAdd(a.queue, 1);
+        if (lane == 0) next = atomicAdd(ka->queue, 1);
         next = gbcast_i<G>(next, 0);
         sys = next;
-        active = sys < a.nsys;
+        active = sys < ka->nsys;
         if (active) load(sys);
     };
     // Prepare an attempt of the current step: trim to t_end, predict b from the last
@@ -222,6 +259,8 @@ __global__ void __launch_bounds__(256) ias15_kernel(IntegrateArgs a) {
 
     if (active) load(sys);
     while (active) {
+        ka = launder(ka);
+        const DevEphem& E = *ka->E;
         if (phase == 0 && t == t_end) { finish(); continue; }  // zero-length span
         // ---- evaluation point of this trip ----
         double xe[3], ve[3], te;
@@ -230,28 +269,21 @@ __global__ void __launch_bounds__(256) ias15_kernel(IntegrateArgs a) {
             for (int c = 0; c < 3; c++) { xe[c] = x0[c]; ve[c] = v0[c]; }
             te = t;
         } else {
-            const double h = cH[n];
+            // predictor weights at spacing h_n (tables: the position and velocity
+            // polynomials' coefficients times powers of h_n)
+            const double* px = cPX + 8 * n;
+            const double* pv = cPV + 7 * n;
+            const double dt2 = dts * dts;
             double s[9];
-            s[0] = dts * h;
-            s[1] = s[0] * s[0] / 2.;
-            s[2] = s[1] * h / 3.;
-            s[3] = s[2] * h / 2.;
-            s[4] = 3. * s[3] * h / 5.;
-            s[5] = 2. * s[4] * h / 3.;
-            s[6] = 5. * s[5] * h / 7.;
-            s[7] = 3. * s[6] * h / 4.;
-            s[8] = 7. * s[7] * h / 9.;
+            s[0] = dts * cH[n];
+#pragma unroll
+            for (int k = 1; k < 9; k++) s[k] = dt2 * px[k - 1];
 #pragma unroll
             for (int c = 0; c < 3; c++)
                 xe[c] = -csx[c] + ((s[8] * b[6][c] + s[7] * b[5][c] + s[6] * b[4][c] + s[5] * b[3][c] + s[4] * b[2][c] +
                                     s[3] * b[1][c] + s[2] * b[0][c] + s[1] * a0[c] + s[0] * v0[c]) + x0[c]);
-            s[1] = s[0] * h / 2.;
-            s[2] = 2. * s[1] * h / 3.;
-            s[3] = 3. * s[2] * h / 4.;
-            s[4] = 4. * s[3] * h / 5.;
-            s[5] = 5. * s[4] * h / 6.;
-            s[6] = 6. * s[5] * h / 7.;
-            s[7] = 7. * s[6] * h / 8.;
+#pragma unroll
+            for (int k = 1; k < 8; k++) s[k] = dts * pv[k - 1];
 #pragma unroll
             for (int c = 0; c < 3; c++)
                 ve[c] = -csv[c] + ((s[7] * b[6][c] + s[6] * b[5][c] + s[5] * b[4][c] + s[4] * b[3][c] + s[3] * b[2][c] +
@@ -264,7 +296,8 @@ __global__ void __launch_bounds__(256) ias15_kernel(IntegrateArgs a) {
         for (int c = 0; c < 3; c++) { xr[c] = gbcast<G>(xe[c], 0); vr[c] = gbcast<G>(ve[c], 0); }
         EvalResult<double> R;
         Jac<TS> J;
-        force_eval<G, Mixed, double>(E, a.fo, lane, te, xr, vr, A, ng, nv > 1, R, J);
+        const ForceOpts fo = ka->fo;
+        force_eval<G, Mixed, double>(E, fo, lane, te, xr, vr, A, ng, nv > 1, R, J);
         double at[3];
         own_accel<double, TS>(lane, R, J, xe, ve, dA, ng, at);
         evals++;
@@ -366,7 +399,7 @@ __global__ void __launch_bounds__(256) ias15_kernel(IntegrateArgs a) {
         }
         // ---- step-size control (real particle only) ----
         double dt_new = dts;
-        if (io.epsilon > 0) {
+        if (ka->io.epsilon > 0) {
             double mak = 0.0, mb6 = 0.0;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -376,9 +409,9 @@ __global__ void __launch_bounds__(256) ias15_kernel(IntegrateArgs a) {
                 if (is_normal(bk) && bk > mb6) mb6 = bk;
             }
             const double err = gbcast<G>(mb6 / mak, 0);
-            if (is_normal(err)) dt_new = pow(io.epsilon / err, 1.0 / 7.0) * dts;
+            if (is_normal(err)) dt_new = pow(ka->io.epsilon / err, 1.0 / 7.0) * dts;
             else dt_new = dts / 0.25;
-            if (fabs(dt_new) < io.min_dt) dt_new = copysign(io.min_dt, dt_new);
+            if (fabs(dt_new) < ka->io.min_dt) dt_new = copysign(ka->io.min_dt, dt_new);
             if (fabs(dt_new / dts) < 0.25) {
                 rejected++;
                 dts = dt_new;
@@ -389,25 +422,33 @@ __global__ void __launch_bounds__(256) ias15_kernel(IntegrateArgs a) {
             if (dt_new / dts > 1. / 0.25) dt_new = dts / 0.25;
         }
         // ---- accept: advance x0, v0 (Kahan) and store br, er ----
-        const double dt2 = dts * dts;
+        // end-of-step weights: the predictor at h = 1 (row 8 of the tables)
+        double wx[8], wv[7];
+        {
+            const double dt2 = dts * dts;
+#pragma unroll
+            for (int k = 0; k < 8; k++) wx[k] = cPX[64 + k] * dt2;
+#pragma unroll
+            for (int k = 0; k < 7; k++) wv[k] = cPV[56 + k] * dts;
+        }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            add_cs(x0[c], csx[c], b[6][c] / 72. * dt2);
-            add_cs(x0[c], csx[c], b[5][c] / 56. * dt2);
-            add_cs(x0[c], csx[c], b[4][c] / 42. * dt2);
-            add_cs(x0[c], csx[c], b[3][c] / 30. * dt2);
-            add_cs(x0[c], csx[c], b[2][c] / 20. * dt2);
-            add_cs(x0[c], csx[c], b[1][c] / 12. * dt2);
-            add_cs(x0[c], csx[c], b[0][c] / 6. * dt2);
-            add_cs(x0[c], csx[c], a0[c] / 2. * dt2);
+            add_cs(x0[c], csx[c], b[6][c] * wx[7]);
+            add_cs(x0[c], csx[c], b[5][c] * wx[6]);
+            add_cs(x0[c], csx[c], b[4][c] * wx[5]);
+            add_cs(x0[c], csx[c], b[3][c] * wx[4]);
+            add_cs(x0[c], csx[c], b[2][c] * wx[3]);
+            add_cs(x0[c], csx[c], b[1][c] * wx[2]);
+            add_cs(x0[c], csx[c], b[0][c] * wx[1]);
+            add_cs(x0[c], csx[c], a0[c] * wx[0]);
             add_cs(x0[c], csx[c], v0[c] * dts);
-            add_cs(v0[c], csv[c], b[6][c] / 8. * dts);
-            add_cs(v0[c], csv[c], b[5][c] / 7. * dts);
-            add_cs(v0[c], csv[c], b[4][c] / 6. * dts);
-            add_cs(v0[c], csv[c], b[3][c] / 5. * dts);
-            add_cs(v0[c], csv[c], b[2][c] / 4. * dts);
-            add_cs(v0[c], csv[c], b[1][c] / 3. * dts);
-            add_cs(v0[c], csv[c], b[0][c] / 2. * dts);
+            add_cs(v0[c], csv[c], b[6][c] * wv[6]);
+            add_cs(v0[c], csv[c], b[5][c] * wv[5]);
+            add_cs(v0[c], csv[c], b[4][c] * wv[4]);
+            add_cs(v0[c], csv[c], b[3][c] * wv[3]);
+            add_cs(v0[c], csv[c], b[2][c] * wv[2]);
+            add_cs(v0[c], csv[c], b[1][c] * wv[1]);
+            add_cs(v0[c], csv[c], b[0][c] * wv[0]);
             add_cs(v0[c], csv[c], a0[c] * dts);
         }
         if (own) {
@@ -428,14 +469,14 @@ __global__ void __launch_bounds__(256) ias15_kernel(IntegrateArgs a) {
         t = last ? t_end : t + dts;
         steps++;
         dt_last = dts;
-        if (!(last && io.epsilon <= 0)) dt = dt_new;
+        if (!(last && ka->io.epsilon <= 0)) dt = dt_new;
         int bad = 0;
 #pragma unroll
         for (int c = 0; c < 3; c++) bad |= !(fabs(x0[c]) < 1e300) || !(fabs(v0[c]) < 1e300);
         bad = gor<G>(own ? bad : 0);
         if (bad) { status = kNonFinite; finish(); continue; }
         if (last) { finish(); continue; }
-        if (steps >= io.max_steps) { status = kMaxSteps; finish(); continue; }
+        if (steps >= ka->io.max_steps) { status = kMaxSteps; finish(); continue; }
         phase = 0;
     }
 }
@@ -467,40 +508,45 @@ hipError_t launch_forces(const ForcesArgs& a, int G, bool mixed, hipStream_t s) 
     }
 }
 
-template <int G, bool M>
-static hipError_t launch_ias15_t(const IntegrateArgs& a, int blocks, int threads, hipStream_t s) {
-    hipLaunchKernelGGL((ias15_kernel<G, M>), dim3(blocks), dim3(threads), 0, s, a);
+template <int G, bool M, int W>
+static const void* ias15_fn() {
+    return (const void*)ias15_kernel<G, M, W>;
+}
+
+static const void* ias15_lookup(int G, bool mixed, int waves) {
+    const int key = (G * 2 + (mixed ? 1 : 0)) * 4 + waves;
+    switch (key) {
+#define CASE(g, m, w) case (g * 2 + m) * 4 + w: return ias15_fn<g, (bool)m, w>();
+    CASE(1, 0, 1) CASE(1, 1, 1) CASE(2, 0, 1) CASE(2, 1, 1) CASE(4, 0, 1) CASE(4, 1, 1) CASE(8, 0, 1) CASE(8, 1, 1)
+    CASE(1, 0, 2) CASE(1, 1, 2) CASE(2, 0, 2) CASE(2, 1, 2) CASE(4, 0, 2) CASE(4, 1, 2) CASE(8, 0, 2) CASE(8, 1, 2)
+#undef CASE
+    default: return nullptr;
+    }
+}
+
+size_t ias15_lds_bytes(int G, int nvec, int threads) {
+    return ((size_t)(threads / G) * ias15_cols_per_group(G, nvec) + 1) * kLdsColWords * sizeof(double);
+}
+
+hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int waves, int blocks, int threads, hipStream_t s) {
+    const void* f = ias15_lookup(G, mixed, waves);
+    if (!f) return hipErrorInvalidValue;
+    const size_t lds = ias15_lds_bytes(G, a.nvec, threads);
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    void* params[] = {(void*)&a};
+    e = hipLaunchKernel(f, dim3(blocks), dim3(threads), params, lds, s);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 
-hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int blocks, int threads, hipStream_t s) {
-    switch (G * 2 + (mixed ? 1 : 0)) {
-    case 2: return launch_ias15_t<1, false>(a, blocks, threads, s);
-    case 3: return launch_ias15_t<1, true>(a, blocks, threads, s);
-    case 4: return launch_ias15_t<2, false>(a, blocks, threads, s);
-    case 5: return launch_ias15_t<2, true>(a, blocks, threads, s);
-    case 8: return launch_ias15_t<4, false>(a, blocks, threads, s);
-    case 9: return launch_ias15_t<4, true>(a, blocks, threads, s);
-    case 16: return launch_ias15_t<8, false>(a, blocks, threads, s);
-    case 17: return launch_ias15_t<8, true>(a, blocks, threads, s);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t ias15_occupancy(int G, bool mixed, int threads, int* blocks_per_cu) {
-    const void* f = nullptr;
-    switch (G * 2 + (mixed ? 1 : 0)) {
-    case 2: f = (const void*)ias15_kernel<1, false>; break;
-    case 3: f = (const void*)ias15_kernel<1, true>; break;
-    case 4: f = (const void*)ias15_kernel<2, false>; break;
-    case 5: f = (const void*)ias15_kernel<2, true>; break;
-    case 8: f = (const void*)ias15_kernel<4, false>; break;
-    case 9: f = (const void*)ias15_kernel<4, true>; break;
-    case 16: f = (const void*)ias15_kernel<8, false>; break;
-    case 17: f = (const void*)ias15_kernel<8, true>; break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, threads, 0);
+hipError_t ias15_occupancy(int G, bool mixed, int waves, int nvec, int threads, int* blocks_per_cu) {
+    const void* f = ias15_lookup(G, mixed, waves);
+    if (!f) return hipErrorInvalidValue;
+    const size_t lds = ias15_lds_bytes(G, nvec, threads);
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, threads, lds);
 }
 
 // Sets the work-queue counter (a kernel, so the sequence can be captured in a graph).

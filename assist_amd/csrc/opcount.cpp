@@ -50,6 +50,8 @@ inline CntF sqrt(CntF a) { g_ops32++; return CntF(::sqrtf(a.v)); }
 inline CntF pow(CntF a, CntF b) { g_ops32++; return CntF(::powf(a.v, b.v)); }
 inline Cnt pow(Cnt a, Cnt b) { g_ops64++; return Cnt(::pow(a.v, b.v)); }
 inline double value_of(Cnt a) { return a.v; }
+inline Cnt inv_sqrt(Cnt a) { g_ops64++; return Cnt(1.0 / ::sqrt(a.v)); }
+inline CntF inv_sqrt(CntF a) { g_ops32++; return CntF(1.0f / ::sqrtf(a.v)); }
 
 #include "force_model.h"
 
@@ -85,6 +87,8 @@ static DevEphem fake_ephem(std::vector<double>& buf) {
     E.c_squared = 3e4; E.over_c_squared = 1.0 / 3e4;
     E.e_cosa = 1; E.e_sina = 0; E.e_cosd = 6e-17; E.e_sind = 1;
     E.s_cosa = 0.27; E.s_sina = -0.96; E.s_cosd = 0.44; E.s_sind = 0.9;
+    E.k_J2 = 1.5 * E.J2E * E.Re_eq * E.Re_eq; E.k_J3 = 2.5 * E.J3E * E.Re_eq * E.Re_eq * E.Re_eq;
+    E.k_J4 = 0.625 * E.J4E * E.Re_eq * E.Re_eq * E.Re_eq * E.Re_eq; E.k_sJ2 = 1.5 * E.J2SUN * E.Rs_eq * E.Rs_eq;
     return E;
 }
 
@@ -118,21 +122,20 @@ struct IasCounts {
     long long correct_comp[8];  // per component at substep n (1..7)
     long long setup_comp;       // per component per step attempt: b from br/er, g from b
     long long accept_comp;      // per component per accepted step: x0, v0 update and br/er
+    long long accept_scalar;    // per accepted step: the end-of-step weights
     long long control;          // per step attempt: step-size control (system)
 };
 
 static IasCounts count_ias15() {
     IasCounts c;
     memset(&c, 0, sizeof c);
-    Cnt dts(1.0), h(0.5), s[9];
+    Cnt dts(1.0), h(0.5), s[9], sv[8];
+    const Cnt px(0.01), pv(0.02);  // table weights (constant memory)
     g_ops64 = 0;
-    s[0] = dts * h; s[1] = s[0] * s[0] / Cnt(2.); s[2] = s[1] * h / Cnt(3.); s[3] = s[2] * h / Cnt(2.);
-    s[4] = Cnt(3.) * s[3] * h / Cnt(5.); s[5] = Cnt(2.) * s[4] * h / Cnt(3.); s[6] = Cnt(5.) * s[5] * h / Cnt(7.);
-    s[7] = Cnt(3.) * s[6] * h / Cnt(4.); s[8] = Cnt(7.) * s[7] * h / Cnt(9.);
-    Cnt sv[8];
-    sv[1] = s[0] * h / Cnt(2.); sv[2] = Cnt(2.) * sv[1] * h / Cnt(3.); sv[3] = Cnt(3.) * sv[2] * h / Cnt(4.);
-    sv[4] = Cnt(4.) * sv[3] * h / Cnt(5.); sv[5] = Cnt(5.) * sv[4] * h / Cnt(6.); sv[6] = Cnt(6.) * sv[5] * h / Cnt(7.);
-    sv[7] = Cnt(7.) * sv[6] * h / Cnt(8.);
+    s[0] = dts * h;
+    Cnt dt2p = dts * dts;
+    for (int k = 1; k < 9; k++) s[k] = dt2p * px;
+    for (int k = 1; k < 8; k++) sv[k] = dts * pv;
     Cnt te = Cnt(0.0) + s[0];
     (void)te;
     c.substep_scalar = g_ops64;
@@ -187,12 +190,19 @@ static IasCounts count_ias15() {
             cs = (t - p) - y;
             p = t;
         };
-        Cnt dt2 = dts * dts;
-        const double kx[8] = {72, 56, 42, 30, 20, 12, 6, 2};
-        for (int k = 6; k >= 0; k--) add_cs(x0, csx, b[k] / Cnt(kx[6 - k]) * dt2);
-        add_cs(x0, csx, a0 / Cnt(2.) * dt2);
+        Cnt wx[8], wv[7];
+        {
+            const long long before = g_ops64;
+            Cnt dt2 = dts * dts;
+            for (int k = 0; k < 8; k++) wx[k] = px * dt2;
+            for (int k = 0; k < 7; k++) wv[k] = pv * dts;
+            c.accept_scalar = g_ops64 - before;
+            g_ops64 = before;
+        }
+        for (int k = 6; k >= 0; k--) add_cs(x0, csx, b[k] * wx[k + 1]);
+        add_cs(x0, csx, a0 * wx[0]);
         add_cs(x0, csx, v0 * dts);
-        for (int k = 6; k >= 0; k--) add_cs(v0, csv, b[k] / Cnt(k + 2.0) * dts);
+        for (int k = 6; k >= 0; k--) add_cs(v0, csv, b[k] * wv[k]);
         add_cs(v0, csv, a0 * dts);
         // er = Q(q) br_old (same arithmetic as the setup's e)
         c.accept_comp = g_ops64 + (c.setup_comp - 7 * 3 - 21 - 21 + 6);  // e only: setup minus b = e+(br-er) and g = Db
@@ -230,7 +240,7 @@ int main() {
     long long corr = 0;
     for (int n = 1; n <= 7; n++) corr += c.correct_comp[n];
     printf("  \"ias15\": {\"substep_scalar\": %lld, \"predict_comp\": %lld, \"correct_comp_7substeps\": %lld, "
-           "\"setup_comp\": %lld, \"accept_comp\": %lld, \"control\": %lld}\n}\n",
-           c.substep_scalar, c.predict_comp, corr, c.setup_comp, c.accept_comp, c.control);
+           "\"setup_comp\": %lld, \"accept_comp\": %lld, \"accept_scalar\": %lld, \"control\": %lld}\n}\n",
+           c.substep_scalar, c.predict_comp, corr, c.setup_comp, c.accept_comp, c.accept_scalar, c.control);
     return 0;
 }

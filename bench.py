@@ -70,7 +70,8 @@ def algorithmic_flops(oc, nvec, ng, mixed, evals, steps, rejected):
     f64 = (ev_tot - ev_ng) * ev0["fp64"] + ev_ng * ev1["fp64"]
     f32 = (ev_tot - ev_ng) * ev0["fp32"] + ev_ng * ev1["fp32"]
     f64 += substeps * (ias["substep_scalar"] + comps * (ias["predict_comp"] + ias["correct_comp_7substeps"] / 7.0))
-    f64 += (st_tot + rej_tot) * (comps * ias["setup_comp"] + ias["control"]) + st_tot * comps * ias["accept_comp"]
+    f64 += (st_tot + rej_tot) * (comps * ias["setup_comp"] + ias["control"])
+    f64 += st_tot * (comps * ias["accept_comp"] + ias.get("accept_scalar", 0))
     return f64, f32
 
 
@@ -126,6 +127,38 @@ def cpu_baseline(paths, cfg, t0, t1, states, params, target_cpu_s=20.0):
                 seconds=el), r, m
 
 
+def copy_calibration(mib):
+    """One plain device-to-device copy of ``mib`` MiB (the runtime's copy kernel reads and writes
+    that many bytes), for scaling the HBM traffic counters of the integration kernel measured in
+    the same profiled process."""
+    from assist_amd import batch
+    n = mib * (1 << 20) // 8
+    src = batch.DeviceArray(np.ones(n))
+    dst = batch.DeviceArray(np.zeros(n))
+    batch.synchronize()
+    dst.copy_from(src)
+    batch.synchronize()
+    print("copy calibration: %d MiB read, %d MiB written" % (mib, mib), file=sys.stderr, flush=True)
+    src.free()
+    dst.free()
+
+
+def measured_traffic(config, precision, n):
+    """HBM bytes per launch of the integration kernel from the committed counter profile
+    (profiles/ias15_kernel_traffic.md), or None when this workload was not profiled."""
+    path = os.path.join(ROOT, "profiles", "ias15_kernel_traffic.md")
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    for line in txt.splitlines():
+        if line.startswith("traffic_json:"):
+            for row in json.loads(line.split(":", 1)[1]):
+                if row["config"] == config and row["precision"] == precision and row["systems"] == n:
+                    return row["bytes_per_launch"]
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,7 +169,11 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override the number of systems")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--waves", type=int, default=0, help="kernel variant: 1 or 2 waves per SIMD (0 = choose)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-kernel-mib", type=int, default=0,
+                    help="also copy this many MiB device-to-device before the timed region, to calibrate the "
+                         "FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: every system is independent, so N GPUs would each take 1/N of the "
@@ -148,6 +185,8 @@ def main():
     if batch.gpu_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_kernel_mib:
+        copy_calibration(args.copy_kernel_mib)
     cfg = dict(CONFIGS[args.config])
     if args.n:
         cfg["n"] = args.n
@@ -158,7 +197,7 @@ def main():
     n, nv = states.shape[0], states.shape[1]
     mixed = args.precision == "mixed"
     bc = batch.BatchConfig.default(precision=1 if mixed else 0, lanes_per_system=args.lanes, block_size=args.block,
-                                   min_dt=cfg["min_dt"], max_steps=200000)
+                                   waves_per_simd=args.waves, min_dt=cfg["min_dt"], max_steps=200000)
 
     d_t0, d_t1 = batch.DeviceArray(t0), batch.DeviceArray(t1)
     d_init = batch.DeviceArray(states)
@@ -199,6 +238,7 @@ def main():
     ach64 = f64 / (kms * 1e-3) / 1e12
     ach32 = f32 / (kms * 1e-3) / 1e12
     frac = ach64 / FP64_PEAK_TFLOPS + ach32 / FP32_PEAK_TFLOPS
+    traffic = measured_traffic(args.config, args.precision, n)
     result = {
         "metric": "particle-days/sec (plain FP64 vs mixed+learned) at <=1 m 10-yr position error",
         "value": value,
@@ -214,11 +254,12 @@ def main():
         "data": "synthetic ephemeris (seed 0) and synthetic particle states",
         "config": {"workload": cfg["workload"], "systems": n, "vectors_per_system": nv, "form": args.precision,
                    "particle_days_per_step": pdays, "lanes_per_system": st.lanes_per_system,
-                   "block_size": st.block_size, "grid_blocks": st.grid_blocks},
+                   "block_size": st.block_size, "grid_blocks": st.grid_blocks, "waves_per_simd": st.waves_per_simd},
         "roofline": {"bound": "fp64-valu", "achieved": ach64 + ach32, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": frac, "traffic": None,
+                     "frac": frac, "traffic": traffic,
                      "kernel": "ias15_kernel", "kernel_ms": kms, "flops_fp64_per_launch": f64,
-                     "flops_fp32_per_launch": f32, "achieved_fp64": ach64, "achieved_fp32": ach32},
+                     "flops_fp32_per_launch": f32, "achieved_fp64": ach64, "achieved_fp32": ach32,
+                     "hbm_gbs": None if traffic is None else traffic / (kms * 1e-3) / 1e9},
         "integration": {"steps_total": int(steps.sum()), "evals_total": int(evals.sum()),
                         "status_nonzero": int((status != 0).sum())},
     }

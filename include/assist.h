@@ -250,12 +250,14 @@ struct assist_batch_config {
     int lanes_per_system; /* 0 = choose (autotuned), else 1, 2, 4 or 8 (>= 1 + n_var) */
     int block_size;       /* 0 = choose, else a multiple of 64 up to 256 */
     int blocks_per_cu;    /* 0 = occupancy limit */
+    int waves_per_simd;   /* 0 = choose (autotuned), else 1 or 2: the kernel variant's register budget */
+    int reserved_;
 };
 
 struct assist_batch_stats {
     long long steps, rejected, evals;
     double kernel_ms;     /* device time of the integration kernel (HIP events) */
-    int lanes_per_system, block_size, grid_blocks;
+    int lanes_per_system, block_size, grid_blocks, waves_per_simd;
 };
 
 void assist_batch_config_default(struct assist_batch_config* cfg);
