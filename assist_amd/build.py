@@ -21,7 +21,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("ASSIST_AMD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip"]
+HIP_SOURCES = ["kernels.hip", "mlp.hip"]
 CPP_SOURCES = ["spk.cpp", "assist_api.cpp", "batch.cpp"]
 HEADERS = ["device_types.h", "force_model.h", "ias15_tables.h", "kernels.h", "internal.h"]
 

@@ -6,7 +6,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "device_types.h"
@@ -292,6 +294,121 @@ void pack(const reb_simulation* sim, const assist_extras* ax, const SimLayout& L
 
 }  // namespace
 
+// ---- launch of the integration kernel and its autotuner ----------------------------------
+
+struct LaunchShape {
+    int G = 8;         // lanes per system
+    int threads = 256; // block size
+    int waves = 1;     // kernel variant (register budget for 1 or 2 waves per SIMD)
+};
+
+constexpr int kTuneMinSystems = 16384;  // smaller batches take the default shape
+constexpr int kTuneSample = 8192;       // systems per trial launch
+constexpr double kTuneSpan = 30.0;      // days integrated per trial system
+
+// Launches set_counter + ias15_kernel on `s` and waits; *ms is the kernel's HIP-event time.
+int run_ias15(IntegrateArgs a, bool mixed, const LaunchShape& sh, int blocks_per_cu_cap, hipStream_t s, float* ms,
+              int* blocks_out) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice") ||
+        !hip_ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute") ||
+        !hip_ok(ias15_occupancy(sh.G, mixed, sh.waves, a.nvec, sh.threads, &per_cu), "occupancy"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    if (per_cu < 1) per_cu = 1;
+    if (blocks_per_cu_cap > 0 && blocks_per_cu_cap < per_cu) per_cu = blocks_per_cu_cap;
+    const long long need_blocks = ((long long)a.nsys * sh.G + sh.threads - 1) / sh.threads;
+    const long long max_blocks = (long long)per_cu * ncu;
+    const int blocks = (int)(need_blocks < max_blocks ? need_blocks : max_blocks);
+    const int total_groups = blocks * sh.threads / sh.G;
+    int rc = ensure_workspace((size_t)blocks * sh.threads * 42);
+    if (rc) return rc;
+    a.scratch = g_ws.scratch;
+    a.queue = g_ws.queue;
+    if (!hip_ok(launch_set_counter(g_ws.queue, total_groups, s), "launch(set_counter)") ||
+        !hip_ok(hipEventRecord(g_ws.ev0, s), "hipEventRecord") ||
+        !hip_ok(launch_ias15(a, sh.G, mixed, sh.waves, blocks, sh.threads, s), "launch(ias15)") ||
+        !hip_ok(hipEventRecord(g_ws.ev1, s), "hipEventRecord") || !hip_ok(hipEventSynchronize(g_ws.ev1), "ias15"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, g_ws.ev0, g_ws.ev1);
+    if (ms) *ms = t;
+    if (blocks_out) *blocks_out = blocks;
+    return ASSIST_SUCCESS;
+}
+
+struct TuneKey {
+    int dev, nvec, mixed, forces, has_params;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(dev, nvec, mixed, forces, has_params) < std::tie(o.dev, o.nvec, o.mixed, o.forces, o.has_params);
+    }
+};
+std::map<TuneKey, LaunchShape> g_tuned;
+
+// Picks lanes per system, block size and kernel variant for this batch's configuration by
+// timing each candidate on copies of the first kTuneSample systems over at most kTuneSpan days
+// (the caller's buffers are not touched); the choice is cached per device and configuration.
+int tuned_shape(const IntegrateArgs& base, bool mixed, hipStream_t s, LaunchShape* out) {
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return ASSIST_BATCH_ERROR_DEVICE;
+    const TuneKey key{dev, base.nvec, mixed ? 1 : 0, base.fo.forces, base.params ? 1 : 0};
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+        *out = it->second;
+        return ASSIST_SUCCESS;
+    }
+    const int m = base.nsys < kTuneSample ? base.nsys : kTuneSample;
+    const size_t sw = (size_t)m * base.nvec * 6;
+    std::vector<double> h0(m), h1(m);
+    DevBuf<double> st, t1;
+    DevBuf<int> status;
+    if (!st.alloc(sw) || !t1.alloc(m) || !status.alloc(m)) return ASSIST_BATCH_ERROR_DEVICE;
+    if (!hip_ok(hipMemcpy(h0.data(), base.t0, m * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy") ||
+        !hip_ok(hipMemcpy(h1.data(), base.t1, m * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    for (int i = 0; i < m; i++) {
+        const double d = h1[i] - h0[i];
+        h1[i] = h0[i] + (d > kTuneSpan ? kTuneSpan : (d < -kTuneSpan ? -kTuneSpan : d));
+    }
+    if (!hip_ok(hipMemcpy(t1.p, h1.data(), m * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    IntegrateArgs a = base;
+    a.nsys = m;
+    a.states = st.p;
+    a.t1 = t1.p;
+    a.status = status.p;
+    a.steps = a.evals = a.rejected = nullptr;
+    a.t_out = nullptr;
+    a.trace = nullptr;
+    a.trace_count = nullptr;
+    LaunchShape best;
+    best.G = lanes_for(base.nvec, 0);
+    float best_ms = 1e30f;
+    for (int G = 1; G <= 8; G <<= 1) {
+        if (G < base.nvec) continue;
+        for (int threads : {256, 128}) {
+            for (int waves : {1, 2}) {
+                LaunchShape sh;
+                sh.G = G;
+                sh.threads = threads;
+                sh.waves = waves;
+                if (!hip_ok(hipMemcpyAsync(st.p, base.states, sw * sizeof(double), hipMemcpyDeviceToDevice, s),
+                            "hipMemcpyAsync"))
+                    return ASSIST_BATCH_ERROR_DEVICE;
+                float ms = 0.f;
+                const int rc = run_ias15(a, mixed, sh, 0, s, &ms, nullptr);
+                if (rc) return rc;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = sh;
+                }
+            }
+        }
+    }
+    g_tuned[key] = best;
+    *out = best;
+    return ASSIST_SUCCESS;
+}
+
 extern "C" {
 
 void assist_batch_config_default(struct assist_batch_config* cfg) {
@@ -388,7 +505,18 @@ int assist_batch_integrate_device(const struct assist_ephem* ephem, const struct
                                   const double* d_t1, double* d_states, const double* d_params, int* d_status,
                                   long long* d_steps, long long* d_evals, void* stream,
                                   struct assist_batch_stats* stats) {
+    return assist_batch_integrate_device_ex(ephem, opts, cfg_in, n, n_var, d_t0, d_t1, d_states, d_params, d_status,
+                                            d_steps, d_evals, stream, stats, nullptr);
+}
+
+int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const struct assist_extras* opts,
+                                     const struct assist_batch_config* cfg_in, int n, int n_var, const double* d_t0,
+                                     const double* d_t1, double* d_states, const double* d_params, int* d_status,
+                                     long long* d_steps, long long* d_evals, void* stream,
+                                     struct assist_batch_stats* stats, const struct assist_batch_extra* ex) {
     if (n < 0 || n_var < 0 || n_var > 7 || !d_t0 || !d_t1 || !d_states || !d_status) return ASSIST_BATCH_ERROR_ARG;
+    if (ex && ex->trace && (!ex->trace_count || ex->trace_capacity < 0 || ex->trace_every < 1 || ex->trace_skip < 0))
+        return ASSIST_BATCH_ERROR_ARG;
     struct assist_batch_config cfg;
     if (cfg_in) cfg = *cfg_in;
     else assist_batch_config_default(&cfg);
@@ -400,26 +528,7 @@ int assist_batch_integrate_device(const struct assist_ephem* ephem, const struct
     rc = get_device_ephem(ephem, &a.E, nullptr);
     if (rc) return rc;
     const int nv = 1 + n_var;
-    const int G = lanes_for(nv, cfg.lanes_per_system);
-    if (G < 0) return ASSIST_BATCH_ERROR_ARG;
     const bool mixed = cfg.precision == ASSIST_PRECISION_MIXED;
-    const int threads = cfg.block_size > 0 ? cfg.block_size : kIas15Block;
-    if (threads % 64 != 0 || threads > kIas15Block) return ASSIST_BATCH_ERROR_ARG;
-    const int waves = cfg.waves_per_simd > 0 ? cfg.waves_per_simd : 1;
-    if (waves != 1 && waves != 2) return ASSIST_BATCH_ERROR_ARG;
-    int dev = 0, ncu = 0, per_cu = 0;
-    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice") ||
-        !hip_ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute") ||
-        !hip_ok(ias15_occupancy(G, mixed, waves, nv, threads, &per_cu), "occupancy"))
-        return ASSIST_BATCH_ERROR_DEVICE;
-    if (per_cu < 1) per_cu = 1;
-    if (cfg.blocks_per_cu > 0 && cfg.blocks_per_cu < per_cu) per_cu = cfg.blocks_per_cu;
-    const long long need_blocks = ((long long)n * G + threads - 1) / threads;
-    const long long max_blocks = (long long)per_cu * ncu;
-    const int blocks = (int)(need_blocks < max_blocks ? need_blocks : max_blocks);
-    const int total_groups = blocks * threads / G;
-    rc = ensure_workspace((size_t)blocks * threads * 42);
-    if (rc) return rc;
     a.io.epsilon = cfg.epsilon;
     a.io.min_dt = cfg.min_dt;
     a.io.dt_initial = cfg.dt_initial;
@@ -433,18 +542,35 @@ int assist_batch_integrate_device(const struct assist_ephem* ephem, const struct
     a.status = d_status;
     a.steps = d_steps;
     a.evals = d_evals;
-    a.scratch = g_ws.scratch;
-    a.queue = g_ws.queue;
+    if (ex) {
+        a.dt0 = ex->dt0;
+        a.bw = ex->bw;
+        a.trace = ex->trace;
+        a.trace_count = ex->trace_count;
+        a.trace_cap = ex->trace_capacity;
+        a.trace_every = ex->trace_every;
+        a.trace_skip = ex->trace_skip;
+    }
     hipStream_t s = (hipStream_t)stream;
-    if (!hip_ok(launch_set_counter(g_ws.queue, total_groups, s), "launch(set_counter)") ||
-        !hip_ok(hipEventRecord(g_ws.ev0, s), "hipEventRecord") ||
-        !hip_ok(launch_ias15(a, G, mixed, waves, blocks, threads, s), "launch(ias15)") ||
-        !hip_ok(hipEventRecord(g_ws.ev1, s), "hipEventRecord") ||
-        !hip_ok(hipEventSynchronize(g_ws.ev1), "ias15"))
-        return ASSIST_BATCH_ERROR_DEVICE;
+    LaunchShape shape;
+    shape.G = lanes_for(nv, cfg.lanes_per_system);
+    if (shape.G < 0) return ASSIST_BATCH_ERROR_ARG;
+    shape.threads = cfg.block_size > 0 ? cfg.block_size : kIas15Block;
+    if (shape.threads % 64 != 0 || shape.threads > kIas15Block || shape.threads % shape.G != 0) return ASSIST_BATCH_ERROR_ARG;
+    shape.waves = cfg.waves_per_simd > 0 ? cfg.waves_per_simd : 1;
+    if (shape.waves != 1 && shape.waves != 2) return ASSIST_BATCH_ERROR_ARG;
+    const bool tune = cfg.lanes_per_system == 0 && cfg.block_size == 0 && cfg.waves_per_simd == 0 &&
+                      n >= kTuneMinSystems;
+    if (tune) {
+        rc = tuned_shape(a, mixed, s, &shape);
+        if (rc) return rc;
+    }
+    float ms = 0.f;
+    int blocks = 0;
+    rc = run_ias15(a, mixed, shape, cfg.blocks_per_cu, s, &ms, &blocks);
+    if (rc) return rc;
+    const int G = shape.G, threads = shape.threads, waves = shape.waves;
     if (stats) {
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, g_ws.ev0, g_ws.ev1);
         stats->kernel_ms = ms;
         stats->lanes_per_system = G;
         stats->block_size = threads;
@@ -566,6 +692,56 @@ void reb_simulation_integrate(struct reb_simulation* sim, double tmax) {
         }
     }
     if (!sim->status) sim->t = tmax;
+}
+
+int assist_step_features(const struct assist_ephem* ephem, int n, const double* d_rows, int stride, float* d_X,
+                         double* d_omega) {
+    if (!ephem || n < 0 || !d_rows || stride < 8 || !d_X) return ASSIST_BATCH_ERROR_ARG;
+    if (n == 0) return ASSIST_SUCCESS;
+    const DevEphem* E = nullptr;
+    int rc = get_device_ephem(ephem, &E, nullptr);
+    if (rc) return rc;
+    if (!hip_ok(launch_features(E, n, d_rows, stride, d_X, d_omega, nullptr), "launch(features)") ||
+        !hip_ok(hipDeviceSynchronize(), "features"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    return ASSIST_SUCCESS;
+}
+
+int assist_batch_predict_start(const struct assist_ephem* ephem, struct assist_mlp* net, int n, int n_var,
+                               const double* d_t0, const double* d_t1, const double* d_states, double min_dt,
+                               int iterations, double* d_dt0, double* d_bw) {
+    if (!ephem || !net || n < 0 || n_var < 0 || n_var > 7 || !d_t0 || !d_t1 || !d_states || !d_dt0 || iterations < 1)
+        return ASSIST_BATCH_ERROR_ARG;
+    if (n == 0) return ASSIST_SUCCESS;
+    const DevEphem* E = nullptr;
+    int rc = get_device_ephem(ephem, &E, nullptr);
+    if (rc) return rc;
+    const int chunk = assist_mlp_max_batch(net);
+    const size_t ns = (size_t)n;
+    DevBuf<double> rows, omega;
+    DevBuf<float> X, Y;
+    if (!rows.alloc(ns * 8) || !omega.alloc(ns) || !X.alloc((size_t)chunk * ASSIST_MLP_IN) ||
+        !Y.alloc((size_t)chunk * ASSIST_MLP_OUT))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    // the first pass sees the last step as 1 day; later passes see the previous prediction
+    if (!hip_ok(launch_start_rows(n, 1 + n_var, d_t0, d_states, 1.0, rows.p, nullptr), "launch(start_rows)"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    for (int it = 0; it < iterations; it++) {
+        for (int off = 0; off < n; off += chunk) {
+            const int m = (n - off < chunk) ? n - off : chunk;
+            if (!hip_ok(launch_features(E, m, rows.p + (size_t)off * 8, 8, X.p, omega.p + off, nullptr), "launch(features)"))
+                return ASSIST_BATCH_ERROR_DEVICE;
+            rc = assist_mlp_forward(net, X.p, m, Y.p, nullptr);
+            if (rc) return rc;
+            const bool last = it == iterations - 1;
+            if (!hip_ok(launch_start_post(m, Y.p, omega.p + off, d_t0 + off, d_t1 + off, min_dt, rows.p + (size_t)off * 8,
+                                          d_dt0 + off, (last && d_bw) ? d_bw + (size_t)off * 21 : nullptr, nullptr),
+                        "launch(start_post)"))
+                return ASSIST_BATCH_ERROR_DEVICE;
+        }
+    }
+    if (!hip_ok(hipDeviceSynchronize(), "predict_start")) return ASSIST_BATCH_ERROR_DEVICE;
+    return ASSIST_SUCCESS;
 }
 
 }  // extern "C"

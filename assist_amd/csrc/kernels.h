@@ -43,7 +43,34 @@ struct IntegrateArgs {
     double* t_out;          // [nsys] or nullptr
     double* scratch;        // [threads][42]: br, er of each lane's last accepted step
     int* queue;             // next system to hand out
+    // learned start (optional): first step and b warm start per system
+    const double* dt0;      // [nsys] first trial step (days, sign ignored) or nullptr
+    const double* bw;       // [nsys][21] b_k,c = bw * |a0| * dt^(k+1) for the real particle, or nullptr
+    // step traces (optional): one record of kTraceWords doubles per sampled accepted step
+    double* trace;
+    int* trace_count;
+    int trace_cap;
+    int trace_every;        // record steps s >= trace_skip with s % trace_every == 0
+    int trace_skip;
+    int pad_;
 };
+
+// Trace record: t, x0[3], v0[3] at the start of the step, the previous step, the accepted
+// step, IAS15's proposal for the next step before the growth cap, a0[3], b[7][3] (k-major).
+constexpr int kTraceWords = 34;
+
+// Features of the learned start: one row per (t, x[3], v[3], dt_last) record with `stride`
+// doubles between rows; writes X[n][16] (float) and, if omega is given, the heliocentric
+// mean motion sqrt(GM_sun / r^3) per row.
+hipError_t launch_features(const DevEphem* E, int n, const double* rows, int stride, float* X, double* omega,
+                           hipStream_t s);
+// Rows [t0, x, v, dt_last] of the real particles of a batch.
+hipError_t launch_start_rows(int n, int nvec, const double* t0, const double* states, double dt_last, double* rows,
+                             hipStream_t s);
+// From network outputs Y[n][32]: dt0 = 10^Y0 (clamped to [dt_min, |t1 - t0|]) written to dt0 and to
+// the rows' dt_last; bw[n][21] = Y[1 + 3k + c] omega^(k+1) / (k+1)! when bw is given.
+hipError_t launch_start_post(int n, const float* Y, const double* omega, const double* t0, const double* t1,
+                             double dt_min, double* rows, double* dt0, double* bw, hipStream_t s);
 
 hipError_t launch_forces(const ForcesArgs& a, int G, bool mixed, hipStream_t s);
 // waves: the kernel variant compiled for 1 or 2 waves per SIMD (register budget 512 or 256).

@@ -176,7 +176,7 @@ ias15_kernel(IntegrateArgs args) {
         }
         t = ka->t0[s];
         t_end = ka->t1[s];
-        dt = ka->io.dt_initial;
+        dt = ka->dt0 ? fabs(ka->dt0[s]) : ka->io.dt_initial;
         if (dt * (t_end - t) < 0) dt = -dt;
         dt_last = 0.0;
         phase = 0;
@@ -232,6 +232,17 @@ ias15_kernel(IntegrateArgs args) {
 #pragma unroll
                     for (int k = 0; k < 7; k++) b[k][c] = e[k] + (br[k] - er[k]);
                 }
+            }
+        } else if (ka->bw && lane == 0) {
+            // learned warm start of the first step: b_k = bw_k |a0| dt^(k+1)
+            const double* w = ka->bw + (size_t)sys * 21;
+            const double amag = sqrt(a0[0] * a0[0] + a0[1] * a0[1] + a0[2] * a0[2]);
+            double hp = dts;
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) b[k][c] = w[3 * k + c] * amag * hp;
+                hp *= dts;
             }
         } else {
 #pragma unroll
@@ -398,7 +409,7 @@ ias15_kernel(IntegrateArgs args) {
             continue;
         }
         // ---- step-size control (real particle only) ----
-        double dt_new = dts;
+        double dt_new = dts, dt_prop = dts;
         if (ka->io.epsilon > 0) {
             double mak = 0.0, mb6 = 0.0;
 #pragma unroll
@@ -419,9 +430,26 @@ ias15_kernel(IntegrateArgs args) {
                 if (!setup_attempt(false)) finish();
                 continue;
             }
+            dt_prop = dt_new;
             if (dt_new / dts > 1. / 0.25) dt_new = dts / 0.25;
         }
         // ---- accept: advance x0, v0 (Kahan) and store br, er ----
+        if (ka->trace && lane == 0 && steps >= ka->trace_skip && steps % ka->trace_every == 0) {
+            const int idx = atomicAdd(ka->trace_count, 1);
+            if (idx < ka->trace_cap) {
+                double* rec = ka->trace + (size_t)idx * kTraceWords;
+                rec[0] = t;
+#pragma unroll
+                for (int c = 0; c < 3; c++) { rec[1 + c] = x0[c]; rec[4 + c] = v0[c]; rec[10 + c] = a0[c]; }
+                rec[7] = dt_last;
+                rec[8] = dts;
+                rec[9] = dt_prop;
+#pragma unroll
+                for (int k = 0; k < 7; k++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) rec[13 + 3 * k + c] = b[k][c];
+            }
+        }
         // end-of-step weights: the predictor at h = 1 (row 8 of the tables)
         double wx[8], wv[7];
         {
@@ -479,6 +507,114 @@ ias15_kernel(IntegrateArgs args) {
         if (steps >= ka->io.max_steps) { status = kMaxSteps; finish(); continue; }
         phase = 0;
     }
+}
+
+// ------------------------------------------------------------------------------------------
+// learned start: features and post-processing
+// ------------------------------------------------------------------------------------------
+// Row r: t = rows[r*stride], x = rows[+1..3], v = rows[+4..6], dt_last = rows[+7].
+// Features: heliocentric position / 3 au, heliocentric velocity * 40 d/au, log10 of the three
+// smallest distances to the 11 planetary bodies (Sun, planets, Moon, Pluto), log10 dt_last.
+__global__ void features_kernel(const DevEphem* __restrict__ Ep, int n, const double* __restrict__ rows, int stride,
+                                float* __restrict__ X, double* __restrict__ omega) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const DevEphem& E = *Ep;
+    const double* row = rows + (size_t)r * stride;
+    const double t = row[0];
+    double pos[12][3];
+    double sunv[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int s = 0; s < kFirstAsteroidSlot; s++) {
+        const ChebOut<double> c = (s == 0) ? cheb_eval<double, true>(E.slot[s], t) : cheb_eval<double, false>(E.slot[s], t);
+        pos[s][0] = c.x; pos[s][1] = c.y; pos[s][2] = c.z;
+        if (s == 0) { sunv[0] = c.vx * E.inv_au; sunv[1] = c.vy * E.inv_au; sunv[2] = c.vz * E.inv_au; }
+    }
+    double d[3] = {1e300, 1e300, 1e300};
+    double xs[3];
+#pragma unroll 1
+    for (int s = 0; s < kFirstAsteroidSlot; s++) {
+        if (s == kSlotEMB) continue;
+        double p[3];
+        const bool em = (s == kSlotEarth || s == kSlotMoon);
+        for (int c = 0; c < 3; c++) p[c] = (em ? pos[s][c] + pos[kSlotEMB][c] : pos[s][c]) * E.inv_au;
+        if (s == 0) for (int c = 0; c < 3; c++) xs[c] = p[c];
+        const double dx = row[1] - p[0], dy = row[2] - p[1], dz = row[3] - p[2];
+        const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+        if (dist < d[0]) { d[2] = d[1]; d[1] = d[0]; d[0] = dist; }
+        else if (dist < d[1]) { d[2] = d[1]; d[1] = dist; }
+        else if (dist < d[2]) d[2] = dist;
+    }
+    float* x = X + (size_t)r * 16;
+    double h[3];
+    for (int c = 0; c < 3; c++) {
+        h[c] = row[1 + c] - xs[c];
+        x[c] = (float)(h[c] / 3.0);
+        x[3 + c] = (float)((row[4 + c] - sunv[c]) * 40.0);
+    }
+    for (int k = 0; k < 3; k++) x[6 + k] = (float)log10(fmax(d[k], 1e-12));
+    x[9] = (float)log10(fmax(fabs(row[7]), 1e-6));
+    for (int k = 10; k < 16; k++) x[k] = 0.f;
+    if (omega) {
+        const double rh = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+        omega[r] = sqrt(E.GM[0] / (rh * rh * rh));
+    }
+}
+
+__global__ void start_rows_kernel(int n, int nvec, const double* __restrict__ t0, const double* __restrict__ states,
+                                  double dt_last, double* __restrict__ rows) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double* row = rows + (size_t)r * 8;
+    const double* st = states + (size_t)r * nvec * 6;
+    row[0] = t0[r];
+    for (int c = 0; c < 6; c++) row[1 + c] = st[c];
+    row[7] = dt_last;
+}
+
+__global__ void start_post_kernel(int n, const float* __restrict__ Y, const double* __restrict__ omega,
+                                  const double* __restrict__ t0, const double* __restrict__ t1, double dt_min,
+                                  double* __restrict__ rows, double* __restrict__ dt0, double* __restrict__ bw) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const float* y = Y + (size_t)r * 32;
+    double dt = pow(10.0, (double)fmin(fmax(y[0], -4.f), 3.f));
+    const double span = fabs(t1[r] - t0[r]);
+    if (span > 0.0 && dt > span) dt = span;
+    if (dt < dt_min) dt = dt_min;
+    dt0[r] = dt;
+    rows[(size_t)r * 8 + 7] = dt;
+    if (bw) {
+        const double w = omega[r];
+        double wp = w, fact = 1.0;
+        for (int k = 0; k < 7; k++) {
+            fact *= (double)(k + 1);
+            for (int c = 0; c < 3; c++) bw[(size_t)r * 21 + 3 * k + c] = (double)y[1 + 3 * k + c] * wp / fact;
+            wp *= w;
+        }
+    }
+}
+
+hipError_t launch_features(const DevEphem* E, int n, const double* rows, int stride, float* X, double* omega,
+                           hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(features_kernel, dim3((n + 127) / 128), dim3(128), 0, s, E, n, rows, stride, X, omega);
+    return hipGetLastError();
+}
+
+hipError_t launch_start_rows(int n, int nvec, const double* t0, const double* states, double dt_last, double* rows,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(start_rows_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, nvec, t0, states, dt_last, rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_start_post(int n, const float* Y, const double* omega, const double* t0, const double* t1,
+                             double dt_min, double* rows, double* dt0, double* bw, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(start_post_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, Y, omega, t0, t1, dt_min, rows,
+                       dt0, bw);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------------------------

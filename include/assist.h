@@ -289,6 +289,26 @@ int assist_batch_integrate_device(const struct assist_ephem* ephem, const struct
                                   long long* d_steps, long long* d_evals, void* stream,
                                   struct assist_batch_stats* stats);
 
+/* Optional inputs and outputs of assist_batch_integrate_device_ex (device pointers). */
+struct assist_batch_extra {
+    const double* dt0;        /* [n] first trial step per system (days; sign taken from t1 - t0), or NULL */
+    const double* bw;         /* [n][21] warm start of the real particle's first step, b[k][c] =
+                                 bw[3k + c] |a0| dt^(k+1), or NULL */
+    double* trace;            /* [capacity][ASSIST_TRACE_WORDS] sampled accepted steps, or NULL */
+    int* trace_count;         /* device counter of records requested (may exceed the capacity) */
+    int trace_capacity;
+    int trace_every;          /* record accepted steps s >= trace_skip with s % trace_every == 0 */
+    int trace_skip;
+    int reserved_;
+};
+
+/* Same as assist_batch_integrate_device, with the learned start and step traces of `ex`. */
+int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const struct assist_extras* opts,
+                                     const struct assist_batch_config* cfg, int n, int n_var, const double* d_t0,
+                                     const double* d_t1, double* d_states, const double* d_params, int* d_status,
+                                     long long* d_steps, long long* d_evals, void* stream,
+                                     struct assist_batch_stats* stats, const struct assist_batch_extra* ex);
+
 /* Device buffer helpers for callers without their own allocator. */
 void* assist_device_malloc(size_t bytes);
 void assist_device_free(void* p);
@@ -296,6 +316,51 @@ int assist_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int assist_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int assist_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int assist_device_synchronize(void);
+
+/* ------------------------------------------------------------------------------------------
+ * Part 3: learned IAS15 start (no counterpart in the reference)
+ *
+ * A multilayer perceptron (16 -> 128 -> 128 -> 128 -> 32, tanh; BF16 weights on the matrix
+ * cores, FP32 accumulation) maps a system's state to the step IAS15 settles on and a warm start
+ * for the first step's b coefficients.  It is trained on the GPU by the entry points below from
+ * step traces of the plain integrator (assist_batch_integrate_device_ex).
+ * ---------------------------------------------------------------------------------------- */
+#define ASSIST_MLP_IN 16
+#define ASSIST_MLP_HIDDEN 128
+#define ASSIST_MLP_OUT 32
+/* Trace record: t, x[3], v[3] at the start of the step, the previous step, the accepted step,
+ * IAS15's next-step proposal before the growth cap, a0[3], b[7][3] (k-major). */
+#define ASSIST_TRACE_WORDS 34
+
+struct assist_mlp;
+int assist_mlp_num_params(void);
+/* Workspace for batches of up to max_batch rows; Glorot-uniform weights from `seed`. */
+struct assist_mlp* assist_mlp_create(int max_batch, unsigned long long seed);
+void assist_mlp_free(struct assist_mlp* net);
+int assist_mlp_max_batch(const struct assist_mlp* net);
+/* FP32 parameters, layer by layer: W[in][out] row-major, then b[out]. */
+int assist_mlp_set_params(struct assist_mlp* net, const float* host);
+int assist_mlp_get_params(const struct assist_mlp* net, float* host);
+/* Per-output weights of the squared-error loss (host array of ASSIST_MLP_OUT). */
+int assist_mlp_set_output_weights(struct assist_mlp* net, const float* host_w);
+/* d_X[M][16] -> d_Y[M][32] (device pointers, M <= max_batch). */
+int assist_mlp_forward(struct assist_mlp* net, const float* d_X, int M, float* d_Y, void* stream);
+/* Forward and backward pass of the loss against d_T[M][32]; gradients stay on the device
+ * (host_grad, if given, receives a copy) and *loss receives the loss. */
+int assist_mlp_loss_grad(struct assist_mlp* net, const float* d_X, const float* d_T, int M, float* loss,
+                         float* host_grad, void* stream);
+/* One Adam step (bias-corrected; step counts from 1) on the gradients of the last loss_grad. */
+int assist_mlp_adam_step(struct assist_mlp* net, float lr, float beta1, float beta2, float eps, int step, void* stream);
+/* Features of rows [t, x[3], v[3], dt_last, ...] (stride doubles apart): d_X[n][16] and, when
+ * d_omega is given, the heliocentric mean motion sqrt(GM_sun / r^3) per row. */
+int assist_step_features(const struct assist_ephem* ephem, int n, const double* d_rows, int stride, float* d_X,
+                         double* d_omega);
+/* First step (d_dt0[n]) and warm start (d_bw[n][21], may be NULL) of a batch from the network,
+ * iterating the last-step input `iterations` times from 1 day; steps are clamped to
+ * [min_dt, |t1 - t0|]. */
+int assist_batch_predict_start(const struct assist_ephem* ephem, struct assist_mlp* net, int n, int n_var,
+                               const double* d_t0, const double* d_t1, const double* d_states, double min_dt,
+                               int iterations, double* d_dt0, double* d_bw);
 
 #ifdef __cplusplus
 }

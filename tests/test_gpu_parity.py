@@ -206,3 +206,18 @@ def test_simulation_dropin_matches_oracle(gpu, ephem, oracle, model):
     assert sim.t == 1000.0
     assert d <= 1.0
     del ex
+
+
+@pytest.mark.parametrize("lanes,block,waves", [(8, 256, 2), (8, 128, 1), (8, 128, 2), (8, 64, 1)])
+def test_integrate_launch_shapes_agree(gpu, ephem, model, lanes, block, waves):
+    """Every launch shape the autotuner may pick (block size, 1- or 2-wave kernel variant) runs
+    the same arithmetic per system: results equal the default shape's bit for bit."""
+    from assist_amd import batch
+    states, _, _ = population(model, 300, seed=41, n_var=6)
+    t0, t1 = np.zeros(300), np.linspace(-400.0, 400.0, 300)
+    ref = batch.integrate(ephem, t0, t1, states, config=batch.BatchConfig.default(min_dt=0.001))
+    cfg = batch.BatchConfig.default(min_dt=0.001, lanes_per_system=lanes, block_size=block, waves_per_simd=waves)
+    out = batch.integrate(ephem, t0, t1, states, config=cfg)
+    assert out.stats.waves_per_simd == waves and out.stats.block_size == block
+    assert (out.status == ref.status).all() and (out.steps == ref.steps).all()
+    assert np.array_equal(out.states, ref.states)
