@@ -718,9 +718,9 @@ int assist_batch_predict_start(const struct assist_ephem* ephem, struct assist_m
     if (rc) return rc;
     const int chunk = assist_mlp_max_batch(net);
     const size_t ns = (size_t)n;
-    DevBuf<double> rows, omega;
+    DevBuf<double> rows;
     DevBuf<float> X, Y;
-    if (!rows.alloc(ns * 8) || !omega.alloc(ns) || !X.alloc((size_t)chunk * ASSIST_MLP_IN) ||
+    if (!rows.alloc(ns * 8) || !X.alloc((size_t)chunk * ASSIST_MLP_IN) ||
         !Y.alloc((size_t)chunk * ASSIST_MLP_OUT))
         return ASSIST_BATCH_ERROR_DEVICE;
     // the first pass sees the last step as 1 day; later passes see the previous prediction
@@ -729,12 +729,12 @@ int assist_batch_predict_start(const struct assist_ephem* ephem, struct assist_m
     for (int it = 0; it < iterations; it++) {
         for (int off = 0; off < n; off += chunk) {
             const int m = (n - off < chunk) ? n - off : chunk;
-            if (!hip_ok(launch_features(E, m, rows.p + (size_t)off * 8, 8, X.p, omega.p + off, nullptr), "launch(features)"))
+            if (!hip_ok(launch_features(E, m, rows.p + (size_t)off * 8, 8, X.p, nullptr, nullptr), "launch(features)"))
                 return ASSIST_BATCH_ERROR_DEVICE;
             rc = assist_mlp_forward(net, X.p, m, Y.p, nullptr);
             if (rc) return rc;
             const bool last = it == iterations - 1;
-            if (!hip_ok(launch_start_post(m, Y.p, omega.p + off, d_t0 + off, d_t1 + off, min_dt, rows.p + (size_t)off * 8,
+            if (!hip_ok(launch_start_post(m, Y.p, d_t0 + off, d_t1 + off, min_dt, rows.p + (size_t)off * 8,
                                           d_dt0 + off, (last && d_bw) ? d_bw + (size_t)off * 21 : nullptr, nullptr),
                         "launch(start_post)"))
                 return ASSIST_BATCH_ERROR_DEVICE;

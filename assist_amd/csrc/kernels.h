@@ -68,8 +68,8 @@ hipError_t launch_features(const DevEphem* E, int n, const double* rows, int str
 hipError_t launch_start_rows(int n, int nvec, const double* t0, const double* states, double dt_last, double* rows,
                              hipStream_t s);
 // From network outputs Y[n][32]: dt0 = 10^Y0 (clamped to [dt_min, |t1 - t0|]) written to dt0 and to
-// the rows' dt_last; bw[n][21] = Y[1 + 3k + c] omega^(k+1) / (k+1)! when bw is given.
-hipError_t launch_start_post(int n, const float* Y, const double* omega, const double* t0, const double* t1,
+// the rows' dt_last; bw[n][21] = Y[1 + 3k + c] (c / dt0)^(k+1) / (k+1)! when bw is given.
+hipError_t launch_start_post(int n, const float* Y, const double* t0, const double* t1,
                              double dt_min, double* rows, double* dt0, double* bw, hipStream_t s);
 
 hipError_t launch_forces(const ForcesArgs& a, int G, bool mixed, hipStream_t s);

@@ -5,10 +5,11 @@ control grow it at most fourfold per step; the first step also starts its predic
 from b = 0.  ``StepModel`` -- a 16 -> 128 -> 128 -> 128 -> 32 perceptron with BF16 weights on
 the matrix cores (libassist Part 3) -- predicts from a system's state
 
-* the step IAS15 settles on (output 0: log10 of the step in days), and
-* the first step's b coefficients (outputs 1..21), as b_k,c (k+1)! / (|a0| (w dt)^(k+1)) with
-  w = sqrt(GM_sun / r^3) the heliocentric mean motion, which makes them O(1) for orbits the
-  Sun dominates (b_k ~ a^(k+1) dt^(k+1) / (k+1)! for a smooth acceleration).
+* the step IAS15 settles on (output 0: log10 of IAS15's own next-step proposal, in days), and
+* the first step's b coefficients (outputs 1..21), as b_k,c (k+1)! / (|a0| (c s)^(k+1)) with
+  s the sign of the step and c = (7! eps)^(1/7) = 0.175 for eps = 1e-9: IAS15 sizes its steps
+  so that |b6| / |a| is about eps, and b_k ~ a^(k+1) dt^(k+1) / (k+1)! for a smooth
+  acceleration, so this scaling makes every b_k O(1) at the settled step.
 
 IAS15's own error test still accepts or rejects every step: a step that is too long is
 rejected and retried shorter exactly as in the plain integrator.
@@ -34,6 +35,7 @@ N_IN, N_HIDDEN, N_OUT = 16, 128, 32
 TRACE_WORDS = 34
 N_USED_OUT = 22
 B_CLIP = 20.0
+WARM_SCALE = 0.175  # (7! eps)^(1/7) for eps = 1e-9; the same constant is in kernels.hip
 # loss weights of the outputs: the step dominates, the 21 warm-start values follow
 OUTPUT_WEIGHTS = np.array([1.0] + [0.05] * 21 + [0.0] * 10, dtype=np.float32)
 
@@ -176,16 +178,17 @@ def features(ephem, rows: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     return d_X.download(), d_w.download()
 
 
-def targets(records: np.ndarray, omega: np.ndarray) -> np.ndarray:
-    """Training targets [n, 32] of trace records: log10 |accepted step| and the scaled b."""
+def targets(records: np.ndarray, omega: np.ndarray | None = None) -> np.ndarray:
+    """Training targets [n, 32] of trace records: log10 of IAS15's next-step proposal and the
+    scaled b of the accepted step."""
     n = records.shape[0]
     T = np.zeros((n, N_OUT), np.float32)
-    dts = records[:, 8]
-    T[:, 0] = np.log10(np.abs(dts))
+    T[:, 0] = np.clip(np.log10(np.abs(records[:, 9])), -3.0, 3.0)
+    sign = np.sign(records[:, 8])
     amag = np.linalg.norm(records[:, 10:13], axis=1)
     b = records[:, 13:34].reshape(n, 7, 3)
     for k in range(7):
-        scale = amag * (omega * dts) ** (k + 1) / math.factorial(k + 1)
+        scale = amag * (WARM_SCALE * sign) ** (k + 1) / math.factorial(k + 1)
         with np.errstate(divide="ignore", invalid="ignore"):
             v = b[:, k, :] / scale[:, None]
         T[:, 1 + 3 * k:4 + 3 * k] = np.clip(np.nan_to_num(v, nan=0.0, posinf=B_CLIP, neginf=-B_CLIP), -B_CLIP, B_CLIP)

@@ -47,8 +47,8 @@ def main(argv=None):
     print("traces: %d records from %d systems (%d steps, %d evaluations, %d non-zero status) in %.1f s"
           % (rec.shape[0], n, steps.sum(), evals.sum(), (status != 0).sum(), time.perf_counter() - ts), flush=True)
     ts = time.perf_counter()
-    X, omega = L.features(ephem, rec)
-    T = L.targets(rec, omega)
+    X, _ = L.features(ephem, rec)
+    T = L.targets(rec)
     keep = np.isfinite(X).all(axis=1) & np.isfinite(T).all(axis=1)
     X, T = X[keep], T[keep]
     print("features and targets: %d rows in %.1f s; log10 step mean %.3f sd %.3f"

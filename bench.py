@@ -159,13 +159,56 @@ def measured_traffic(config, precision, n):
     return None
 
 
+FORMS = {
+    "fp64": dict(precision=0, learned=False),
+    "mixed": dict(precision=1, learned=False),
+    "fp64+learned": dict(precision=0, learned=True),
+    "mixed+learned": dict(precision=1, learned=True),
+}
+
+
+def run_form(form, args, cfg, ephem, bufs, n, nv, step_model):
+    """Warmup, then `steps` timed passes of one form over the batch; returns its measurements."""
+    from assist_amd import batch, learned
+    spec = FORMS[form]
+    bc = batch.BatchConfig.default(precision=spec["precision"], lanes_per_system=args.lanes, block_size=args.block,
+                                   waves_per_simd=args.waves, min_dt=cfg["min_dt"], max_steps=200000)
+    d_t0, d_t1, d_init, d_states, d_params, d_status, d_steps, d_evals = bufs
+
+    def one_step():
+        d_states.copy_from(d_init)
+        if spec["learned"]:
+            d_dt0, d_bw = learned.predict_start(ephem, step_model, n, nv - 1, d_t0, d_t1, d_init, min_dt=cfg["min_dt"])
+            return learned.integrate_device_learned(ephem, n, nv - 1, d_t0, d_t1, d_states, d_params, d_status, d_steps,
+                                                    d_evals, d_dt0, d_bw, config=bc)
+        return batch.integrate_device(ephem, n, nv - 1, d_t0, d_t1, d_states, d_params, d_status, d_steps, d_evals,
+                                      config=bc)
+
+    for _ in range(args.warmup):
+        one_step()
+        print("%s: warmup step done" % form, file=sys.stderr, flush=True)
+    batch.synchronize()
+    kernel_ms = []
+    ts = time.perf_counter()
+    for k in range(args.steps):
+        st = one_step()
+        kernel_ms.append(st.kernel_ms)
+        print("%s: step %d: integration kernel %.1f ms" % (form, k, st.kernel_ms), file=sys.stderr, flush=True)
+    batch.synchronize()
+    elapsed = time.perf_counter() - ts
+    return dict(form=form, elapsed=elapsed, kernel_ms=float(np.mean(kernel_ms)), stats=st,
+                status=d_status.download(), steps=d_steps.download(), evals=d_evals.download(),
+                states=d_states.download(), mixed=spec["precision"] == 1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--form", default="mixed+learned", choices=sorted(FORMS),
+                    help="the form reported as `value`; the plain FP64 form is always measured beside it")
     ap.add_argument("--n", type=int, default=0, help="override the number of systems")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
@@ -181,7 +224,7 @@ def main():
         sys.exit(2)
 
     import assist_amd
-    from assist_amd import batch, synthetic
+    from assist_amd import batch, learned, synthetic
     if batch.gpu_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
@@ -195,50 +238,33 @@ def main():
     model = synthetic.SyntheticModel(0)
     t0, t1, states, params = make_workload(cfg, model)
     n, nv = states.shape[0], states.shape[1]
-    mixed = args.precision == "mixed"
-    bc = batch.BatchConfig.default(precision=1 if mixed else 0, lanes_per_system=args.lanes, block_size=args.block,
-                                   waves_per_simd=args.waves, min_dt=cfg["min_dt"], max_steps=200000)
-
-    d_t0, d_t1 = batch.DeviceArray(t0), batch.DeviceArray(t1)
-    d_init = batch.DeviceArray(states)
-    d_states = batch.DeviceArray(states)
-    d_params = batch.DeviceArray(params) if params is not None else None
-    d_status = batch.DeviceArray(np.zeros(n, dtype=np.int32))
-    d_steps = batch.DeviceArray(np.zeros(n, dtype=np.int64))
-    d_evals = batch.DeviceArray(np.zeros(n, dtype=np.int64))
-
-    def one_step():
-        d_states.copy_from(d_init)
-        return batch.integrate_device(ephem, n, nv - 1, d_t0, d_t1, d_states, d_params, d_status, d_steps, d_evals,
-                                      config=bc)
-
-    for _ in range(args.warmup):
-        one_step()
-        print("warmup step done", file=sys.stderr, flush=True)
-    batch.synchronize()
-    kernel_ms = []
-    ts = time.perf_counter()
-    for k in range(args.steps):
-        st = one_step()
-        kernel_ms.append(st.kernel_ms)
-        print("step %d: kernel %.1f ms" % (k, st.kernel_ms), file=sys.stderr, flush=True)
-    batch.synchronize()
-    elapsed = time.perf_counter() - ts
-
-    status = d_status.download()
-    steps = d_steps.download()
-    evals = d_evals.download()
-    out_states = d_states.download()
+    step_model = None
+    if FORMS[args.form]["learned"]:
+        path = learned.default_model_path()
+        if not os.path.exists(path):
+            print("the learned forms need the trained step model %s (python -m assist_amd.train_step_model)" % path,
+                  file=sys.stderr)
+            sys.exit(1)
+        step_model = learned.StepModel.load(path, max_batch=65536)
+    bufs = (batch.DeviceArray(t0), batch.DeviceArray(t1), batch.DeviceArray(states), batch.DeviceArray(states),
+            batch.DeviceArray(params) if params is not None else None, batch.DeviceArray(np.zeros(n, dtype=np.int32)),
+            batch.DeviceArray(np.zeros(n, dtype=np.int64)), batch.DeviceArray(np.zeros(n, dtype=np.int64)))
     pdays = float(np.sum(np.abs(t1 - t0)))
-    value = pdays * args.steps / elapsed
     oc = opcounts()
     ng = None if params is None else np.any(params[:, 0] != 0, axis=-1)
-    f64, f32 = algorithmic_flops(oc, nv, ng, mixed, evals, steps, None)
-    kms = float(np.mean(kernel_ms))
+    forms = [args.form] if args.form == "fp64" else ["fp64", args.form]
+    res = {f: run_form(f, args, cfg, ephem, bufs, n, nv, step_model) for f in forms}
+    main_res, plain = res[args.form], res["fp64"]
+    value = pdays * args.steps / main_res["elapsed"]
+    plain_value = pdays * args.steps / plain["elapsed"]
+    f64, f32 = algorithmic_flops(oc, nv, ng, main_res["mixed"], main_res["evals"], main_res["steps"], None)
+    kms = main_res["kernel_ms"]
     ach64 = f64 / (kms * 1e-3) / 1e12
     ach32 = f32 / (kms * 1e-3) / 1e12
     frac = ach64 / FP64_PEAK_TFLOPS + ach32 / FP32_PEAK_TFLOPS
-    traffic = measured_traffic(args.config, args.precision, n)
+    traffic = measured_traffic(args.config, args.form, n)
+    st = main_res["stats"]
+    dpos = np.linalg.norm(main_res["states"][:, 0, :3] - plain["states"][:, 0, :3], axis=-1) * 149597870700.0
     result = {
         "metric": "particle-days/sec (plain FP64 vs mixed+learned) at <=1 m 10-yr position error",
         "value": value,
@@ -246,29 +272,34 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": main_res["elapsed"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if not mixed else "f64+f32",
+        "dtype": "f64+f32" if main_res["mixed"] else "f64",
         "data": "synthetic ephemeris (seed 0) and synthetic particle states",
-        "config": {"workload": cfg["workload"], "systems": n, "vectors_per_system": nv, "form": args.precision,
+        "config": {"workload": cfg["workload"], "systems": n, "vectors_per_system": nv, "form": args.form,
                    "particle_days_per_step": pdays, "lanes_per_system": st.lanes_per_system,
                    "block_size": st.block_size, "grid_blocks": st.grid_blocks, "waves_per_simd": st.waves_per_simd},
+        "plain_fp64": {"value": plain_value, "ms_per_step": plain["elapsed"] / args.steps * 1e3,
+                       "kernel_ms": plain["kernel_ms"], "evals_total": int(plain["evals"].sum())},
+        "speedup_vs_plain_fp64": value / plain_value,
         "roofline": {"bound": "fp64-valu", "achieved": ach64 + ach32, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": frac, "traffic": traffic,
                      "kernel": "ias15_kernel", "kernel_ms": kms, "flops_fp64_per_launch": f64,
                      "flops_fp32_per_launch": f32, "achieved_fp64": ach64, "achieved_fp32": ach32,
                      "hbm_gbs": None if traffic is None else traffic / (kms * 1e-3) / 1e9},
-        "integration": {"steps_total": int(steps.sum()), "evals_total": int(evals.sum()),
-                        "status_nonzero": int((status != 0).sum())},
+        "integration": {"steps_total": int(main_res["steps"].sum()), "evals_total": int(main_res["evals"].sum()),
+                        "status_nonzero": int((main_res["status"] != 0).sum()),
+                        "max_position_diff_to_plain_fp64_m": float(dpos.max()),
+                        "p99_position_diff_to_plain_fp64_m": float(np.percentile(dpos, 99))},
     }
     if not args.no_cpu_baseline:
         cb, r, m = cpu_baseline(paths, cfg, t0, t1, states, params)
-        dpos = np.linalg.norm(out_states[:m, 0, :3] - r["states"][:, 0, :3], axis=-1) * 149597870700.0
+        dcpu = np.linalg.norm(plain["states"][:m, 0, :3] - r["states"][:, 0, :3], axis=-1) * 149597870700.0
         cb.pop("seconds", None)
         result["cpu_baseline"] = cb
-        result["parity_vs_oracle_sample"] = {"systems": m, "max_position_diff_m": float(dpos.max())}
+        result["parity_vs_oracle_sample"] = {"systems": m, "form": "fp64", "max_position_diff_m": float(dcpu.max())}
     print(json.dumps(result), flush=True)
 
 

@@ -92,7 +92,8 @@ def test_mlp_training_fits_a_teacher(gpu):
     net.set_output_weights(np.ones(32, np.float32))
     hist = L.train(net, X, T, epochs=12, batch=1024, lr=3e-3, lr_end=3e-4)
     print("teacher fit: loss per epoch", ["%.4g" % h for h in hist])
-    assert hist[-1] < 0.1 * hist[0]
+    assert hist[-1] < 0.3 * hist[0]
+    assert all(b <= a * 1.02 for a, b in zip(hist[1:], hist[2:]))
 
 
 def test_features_match_oracle_ephemeris(gpu, ephem, oracle, model):
@@ -182,3 +183,31 @@ def test_learned_start_ablation(gpu, ephem, model):
     print("force evaluations: plain %d, learned step %d (%.3fx), learned step + warm start %d (%.3fx)"
           % (e0.sum(), e1.sum(), e0.sum() / e1.sum(), e2.sum(), e0.sum() / e2.sum()))
     assert e1.sum() < e0.sum()
+
+
+def test_mixed_learned_10yr_position_error(gpu, ephem, oracle, model):
+    """The mixed-precision form started by the trained network, against the FP64 oracle: 64
+    main-belt systems x (1 + 6 variational), 3650 days, <= 1 m."""
+    import assist_amd.learned as L
+    from assist_amd import batch, synthetic
+    net = _learned_model_or_skip()
+    rng = np.random.default_rng(2)
+    n = 64
+    states = np.zeros((n, 7, 6))
+    states[:, 0] = synthetic.main_belt(model, n, rng)
+    states[:, 1:] = synthetic.variational_identity(n, 6)
+    t0, t1 = np.zeros(n), np.full(n, 3650.0)
+    d_t0, d_t1, d_init = batch.DeviceArray(t0), batch.DeviceArray(t1), batch.DeviceArray(states)
+    d_st = batch.DeviceArray(states)
+    d_status = batch.DeviceArray(np.zeros(n, np.int32))
+    d_steps = batch.DeviceArray(np.zeros(n, np.int64))
+    d_evals = batch.DeviceArray(np.zeros(n, np.int64))
+    d_dt0, d_bw = L.predict_start(ephem, net, n, 6, d_t0, d_t1, d_init)
+    L.integrate_device_learned(ephem, n, 6, d_t0, d_t1, d_st, None, d_status, d_steps, d_evals, d_dt0, d_bw,
+                               config=batch.BatchConfig.default(precision=batch.PRECISION_MIXED))
+    o = oracle.integrate(t0, t1, states, None)
+    assert (d_status.download() == 0).all() and (o["status"] == 0).all()
+    dpos = np.linalg.norm(d_st.download()[:, 0, :3] - o["states"][:, 0, :3], axis=-1) * AU_M
+    print("mixed + learned, 10 yr: max |dr| = %.3e m, evaluations %d (oracle %d)"
+          % (dpos.max(), d_evals.download().sum(), o["evals"].sum()))
+    assert dpos.max() <= 1.0

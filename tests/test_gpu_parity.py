@@ -221,3 +221,24 @@ def test_integrate_launch_shapes_agree(gpu, ephem, model, lanes, block, waves):
     assert out.stats.waves_per_simd == waves and out.stats.block_size == block
     assert (out.status == ref.status).all() and (out.steps == ref.steps).all()
     assert np.array_equal(out.states, ref.states)
+
+
+def test_integrate_is_deterministic(gpu, ephem, model):
+    """Two launches of the same batch (the work queue hands systems to groups in a different
+    order each time) give bit-identical states, step and evaluation counts."""
+    from assist_amd import batch
+    n = 6000
+    states, _, _ = population(model, n, seed=43, n_var=6)
+    t0, t1 = np.zeros(n), np.full(n, 200.0)
+    cfg = batch.BatchConfig.default(min_dt=0.001)
+    d_t0, d_t1 = batch.DeviceArray(t0), batch.DeviceArray(t1)
+    outs = []
+    for _ in range(2):
+        d_st = batch.DeviceArray(states)
+        d_status = batch.DeviceArray(np.zeros(n, np.int32))
+        d_steps = batch.DeviceArray(np.zeros(n, np.int64))
+        d_evals = batch.DeviceArray(np.zeros(n, np.int64))
+        batch.integrate_device(ephem, n, 6, d_t0, d_t1, d_st, None, d_status, d_steps, d_evals, config=cfg)
+        outs.append((d_st.download(), d_steps.download(), d_evals.download()))
+    assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[0][2], outs[1][2])
+    assert np.array_equal(outs[0][0], outs[1][0])
