@@ -10,9 +10,15 @@ namespace assist_gpu {
 constexpr int kIas15Block = 256;
 // Doubles per LDS column of the integration kernel (b and g, 42 words, padded to 43).
 constexpr int kLdsColWords = 43;
-// LDS columns per lane group: one per owned vector; 8-lane groups with at most 7 vectors
-// (a particle and its 6 variational particles) pack 7 columns.
-__host__ __device__ constexpr int ias15_cols_per_group(int G, int nvec) { return (G == 8 && nvec <= 7) ? 7 : G; }
+// LDS columns per lane group: one per lane; the 2-wave variant packs the 7 vectors of 8-lane
+// groups into 7 columns (lanes that own no vector share one spare column) so that two blocks fit
+// a CU's LDS.
+__host__ __device__ constexpr int ias15_cols_per_group(int G, int nvec, int waves) {
+    return (waves == 2 && G == 8 && nvec <= 7) ? 7 : G;
+}
+// IAS15 tables staged in LDS ahead of the columns (doubles): spacings h (8), predictor weights
+// (9 x 8 and 9 x 7), 1/(h_i - h_j) (28), Newton-to-power weights (21).
+constexpr int kTabH = 0, kTabPX = 8, kTabPV = 80, kTabRRI = 143, kTabC = 171, kTabWords = 192;
 
 struct ForcesArgs {
     const DevEphem* E;
@@ -76,7 +82,7 @@ hipError_t launch_forces(const ForcesArgs& a, int G, bool mixed, hipStream_t s);
 // waves: the kernel variant compiled for 1 or 2 waves per SIMD (register budget 512 or 256).
 hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int waves, int blocks, int threads, hipStream_t s);
 hipError_t ias15_occupancy(int G, bool mixed, int waves, int nvec, int threads, int* blocks_per_cu);
-size_t ias15_lds_bytes(int G, int nvec, int threads);
+size_t ias15_lds_bytes(int G, int nvec, int threads, int waves);
 hipError_t launch_set_counter(int* q, int v, hipStream_t s);
 
 }  // namespace assist_gpu

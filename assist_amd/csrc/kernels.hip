@@ -142,9 +142,22 @@ ias15_kernel(IntegrateArgs args) {
     // b and g (7 x 3 each per vector) live in LDS, one column per owning lane, to relieve
     // register pressure; lanes that own no vector share one spare column (their values are
     // never used), which is what lets 7-vector systems on 8-lane groups fit two blocks per CU.
-    extern __shared__ double s_cols[];
-    const int cols_per_group = ias15_cols_per_group(G, nv);
-    const int col = own ? (int)(threadIdx.x / G) * cols_per_group + lane : (int)(blockDim.x / G) * cols_per_group;
+    extern __shared__ double s_lds[];
+    double* const tab = s_lds;
+    double* const s_cols = s_lds + kTabWords;
+    for (int i = threadIdx.x; i < kTabWords; i += blockDim.x) {
+        double v;
+        if (i < kTabPX) v = cH[i - kTabH];
+        else if (i < kTabPV) v = cPX[i - kTabPX];
+        else if (i < kTabRRI) v = cPV[i - kTabPV];
+        else if (i < kTabC) v = cRRI[i - kTabRRI];
+        else v = cC[i - kTabC];
+        tab[i] = v;
+    }
+    __syncthreads();
+    const int cols_per_group = ias15_cols_per_group(G, nv, W);
+    const int col = (own || cols_per_group == G) ? (int)(threadIdx.x / G) * cols_per_group + lane
+                                                 : (int)(blockDim.x / G) * cols_per_group;
     const LdsArr b{s_cols + (size_t)col * kLdsColWords}, g{s_cols + (size_t)col * kLdsColWords + 21};
     double A[3], dA[3];
     double t = 0, t_end = 0, dt = 0, dts = 0, dt_last = 0, pc_err_last = 0;
@@ -282,11 +295,11 @@ ias15_kernel(IntegrateArgs args) {
         } else {
             // predictor weights at spacing h_n (tables: the position and velocity
             // polynomials' coefficients times powers of h_n)
-            const double* px = cPX + 8 * n;
-            const double* pv = cPV + 7 * n;
+            const double* px = tab + kTabPX + 8 * n;
+            const double* pv = tab + kTabPV + 7 * n;
             const double dt2 = dts * dts;
             double s[9];
-            s[0] = dts * cH[n];
+            s[0] = dts * tab[kTabH + n];
 #pragma unroll
             for (int k = 1; k < 9; k++) s[k] = dt2 * px[k - 1];
 #pragma unroll
@@ -320,83 +333,28 @@ ias15_kernel(IntegrateArgs args) {
             if (!setup_attempt(true)) { finish(); continue; }
             continue;
         }
-        // ---- corrector: improve g_n and b ----
+        // ---- corrector: improve g_{n-1} and b_0..b_{n-1}.  One code path for every substep
+        //      (the groups of a wave are at different substeps), tables from LDS. ----
         double maxak = 0.0, maxdb6 = 0.0;
+        {
+            const double* rri = tab + kTabRRI + n * (n - 1) / 2;      // 1 / (h_n - h_j), j = 0..n-1
+            const double* cc = tab + kTabC + (n - 1) * (n - 2) / 2;   // b_k += cc[k] dg_{n-1}, k < n-1
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const double gk = at[c] - a0[c];
-            double tmp;
-            switch (n) {
-            case 1:
-                tmp = g[0][c];
-                g[0][c] = gk * cRRI[0];
-                b[0][c] += g[0][c] - tmp;
-                break;
-            case 2:
-                tmp = g[1][c];
-                g[1][c] = (gk * cRRI[1] - g[0][c]) * cRRI[2];
-                tmp = g[1][c] - tmp;
-                b[0][c] += tmp * cC[0];
-                b[1][c] += tmp;
-                break;
-            case 3:
-                tmp = g[2][c];
-                g[2][c] = ((gk * cRRI[3] - g[0][c]) * cRRI[4] - g[1][c]) * cRRI[5];
-                tmp = g[2][c] - tmp;
-                b[0][c] += tmp * cC[1];
-                b[1][c] += tmp * cC[2];
-                b[2][c] += tmp;
-                break;
-            case 4:
-                tmp = g[3][c];
-                g[3][c] = (((gk * cRRI[6] - g[0][c]) * cRRI[7] - g[1][c]) * cRRI[8] - g[2][c]) * cRRI[9];
-                tmp = g[3][c] - tmp;
-                b[0][c] += tmp * cC[3];
-                b[1][c] += tmp * cC[4];
-                b[2][c] += tmp * cC[5];
-                b[3][c] += tmp;
-                break;
-            case 5:
-                tmp = g[4][c];
-                g[4][c] = ((((gk * cRRI[10] - g[0][c]) * cRRI[11] - g[1][c]) * cRRI[12] - g[2][c]) * cRRI[13] - g[3][c]) * cRRI[14];
-                tmp = g[4][c] - tmp;
-                b[0][c] += tmp * cC[6];
-                b[1][c] += tmp * cC[7];
-                b[2][c] += tmp * cC[8];
-                b[3][c] += tmp * cC[9];
-                b[4][c] += tmp;
-                break;
-            case 6:
-                tmp = g[5][c];
-                g[5][c] = (((((gk * cRRI[15] - g[0][c]) * cRRI[16] - g[1][c]) * cRRI[17] - g[2][c]) * cRRI[18] - g[3][c]) * cRRI[19] -
-                           g[4][c]) * cRRI[20];
-                tmp = g[5][c] - tmp;
-                b[0][c] += tmp * cC[10];
-                b[1][c] += tmp * cC[11];
-                b[2][c] += tmp * cC[12];
-                b[3][c] += tmp * cC[13];
-                b[4][c] += tmp * cC[14];
-                b[5][c] += tmp;
-                break;
-            default:
-                tmp = g[6][c];
-                g[6][c] = ((((((gk * cRRI[21] - g[0][c]) * cRRI[22] - g[1][c]) * cRRI[23] - g[2][c]) * cRRI[24] - g[3][c]) * cRRI[25] -
-                            g[4][c]) * cRRI[26] - g[5][c]) * cRRI[27];
-                tmp = g[6][c] - tmp;
-                b[0][c] += tmp * cC[15];
-                b[1][c] += tmp * cC[16];
-                b[2][c] += tmp * cC[17];
-                b[3][c] += tmp * cC[18];
-                b[4][c] += tmp * cC[19];
-                b[5][c] += tmp * cC[20];
-                b[6][c] += tmp;
-                if (own) {
+            for (int c = 0; c < 3; c++) {
+                // ((gk r_0 - g_0) r_1 - g_1) r_2 ..., each step one fused multiply-add
+                double val = at[c] - a0[c];
+                for (int k = 0; k < n - 1; k++) val = fma(val, rri[k], -g[k][c]);
+                val *= rri[n - 1];
+                const double tmp = val - g[n - 1][c];
+                g[n - 1][c] = val;
+                for (int k = 0; k < n - 1; k++) b[k][c] += tmp * cc[k];
+                b[n - 1][c] += tmp;
+                if (n == 7 && own) {
                     const double ak = fabs(at[c]);
                     if (is_normal(ak) && ak > maxak) maxak = ak;
                     const double db = fabs(tmp);
                     if (is_normal(db) && db > maxdb6) maxdb6 = db;
                 }
-                break;
             }
         }
         if (n < 7) { n++; continue; }
@@ -663,14 +621,15 @@ static const void* ias15_lookup(int G, bool mixed, int waves) {
     }
 }
 
-size_t ias15_lds_bytes(int G, int nvec, int threads) {
-    return ((size_t)(threads / G) * ias15_cols_per_group(G, nvec) + 1) * kLdsColWords * sizeof(double);
+size_t ias15_lds_bytes(int G, int nvec, int threads, int waves) {
+    return (kTabWords + ((size_t)(threads / G) * ias15_cols_per_group(G, nvec, waves) + 1) * kLdsColWords) *
+           sizeof(double);
 }
 
 hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int waves, int blocks, int threads, hipStream_t s) {
     const void* f = ias15_lookup(G, mixed, waves);
     if (!f) return hipErrorInvalidValue;
-    const size_t lds = ias15_lds_bytes(G, a.nvec, threads);
+    const size_t lds = ias15_lds_bytes(G, a.nvec, threads, waves);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     void* params[] = {(void*)&a};
@@ -682,7 +641,7 @@ hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int waves, in
 hipError_t ias15_occupancy(int G, bool mixed, int waves, int nvec, int threads, int* blocks_per_cu) {
     const void* f = ias15_lookup(G, mixed, waves);
     if (!f) return hipErrorInvalidValue;
-    const size_t lds = ias15_lds_bytes(G, nvec, threads);
+    const size_t lds = ias15_lds_bytes(G, nvec, threads, waves);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, threads, lds);
