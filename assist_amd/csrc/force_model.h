@@ -94,20 +94,40 @@ AHD double gmax(double v) {
 #endif
     return v;
 }
-template <int G, typename T>
-AHD T gbcast(T v, int src) {
+// Broadcast from lane SRC of each group: a quad_perm that gives every lane of a quad that quad's
+// lane SRC % 4, then (8-lane groups) row_half_mirror to fetch the other quad's copy for the
+// lanes whose quad does not hold SRC.
 #if defined(__HIP_DEVICE_COMPILE__)
-    if constexpr (G > 1) return __shfl(v, src, G);
+constexpr int dpp_quad(int a, int b, int c, int d) { return a | (b << 2) | (c << 4) | (d << 6); }
+template <int G, int SRC, typename T>
+__device__ __forceinline__ T dpp_bcast(T v) {
+    static_assert(SRC >= 0 && SRC < G, "source lane");
+    if constexpr (G == 2) {
+        return dpp<dpp_quad(SRC, SRC, 2 + SRC, 2 + SRC)>(v);
+    } else if constexpr (G == 4) {
+        return dpp<dpp_quad(SRC, SRC, SRC, SRC)>(v);
+    } else {
+        constexpr int q = SRC & 3;
+        const T v1 = dpp<dpp_quad(q, q, q, q)>(v);
+        const T v2 = dpp<kDppHalfMirror>(v1);
+        const bool upper = (threadIdx.x & 4) != 0;
+        return (upper == (SRC >= 4)) ? v1 : v2;
+    }
+}
 #endif
-    (void)src;
+
+template <int G, int SRC, typename T>
+AHD T gbcast(T v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (G > 1) return dpp_bcast<G, SRC>(v);
+#endif
     return v;
 }
-template <int G>
-AHD int gbcast_i(int v, int src) {
+template <int G, int SRC>
+AHD int gbcast_i(int v) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    if constexpr (G > 1) return __shfl(v, src, G);
+    if constexpr (G > 1) return __float_as_int(dpp_bcast<G, SRC>(__int_as_float(v)));
 #endif
-    (void)src;
     return v;
 }
 template <int G>
@@ -609,9 +629,9 @@ AHD void force_eval(const DevEphem& E, const ForceOpts& O, int lane, D t, const 
     const D inv_au = D(E.inv_au);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        sun[c] = gbcast<G>(px[0][c] * inv_au, 0);
-        sunv[c] = gbcast<G>(sunv_l[c] * inv_au, 0);
-        emb[c] = gbcast<G>(px[kSlotEMB / G][c], kSlotEMB % G);
+        sun[c] = gbcast<G, 0>(px[0][c] * inv_au);
+        sunv[c] = gbcast<G, 0>(sunv_l[c] * inv_au);
+        emb[c] = gbcast<G, kSlotEMB % G>(px[kSlotEMB / G][c]);
     }
 #pragma unroll
     for (int r = 0; r < NR; r++) {
@@ -628,7 +648,7 @@ AHD void force_eval(const DevEphem& E, const ForceOpts& O, int lane, D t, const 
     }
     D earth[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) earth[c] = gbcast<G>(px[kSlotEarth / G][c], kSlotEarth % G);
+    for (int c = 0; c < 3; c++) earth[c] = gbcast<G, kSlotEarth % G>(px[kSlotEarth / G][c]);
 
     // 3. per-body point masses, Jacobian and EIH planet sums on the owning lane
     D ad[3] = {D(0.0), D(0.0), D(0.0)};  // direct acceleration, FP64 bodies

@@ -212,7 +212,7 @@ ias15_kernel(IntegrateArgs args) {
         }
         int next = 0;
         if (lane == 0) next = atomicAdd(ka->queue, 1);
-        next = gbcast_i<G>(next, 0);
+        next = gbcast_i<G, 0>(next);
         sys = next;
         active = sys < ka->nsys;
         if (active) load(sys);
@@ -317,7 +317,7 @@ ias15_kernel(IntegrateArgs args) {
         // ---- force evaluation (whole group) ----
         double xr[3], vr[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) { xr[c] = gbcast<G>(xe[c], 0); vr[c] = gbcast<G>(ve[c], 0); }
+        for (int c = 0; c < 3; c++) { xr[c] = gbcast<G, 0>(xe[c]); vr[c] = gbcast<G, 0>(ve[c]); }
         EvalResult<double> R;
         Jac<TS> J;
         const ForceOpts fo = ka->fo;
@@ -333,22 +333,42 @@ ias15_kernel(IntegrateArgs args) {
             if (!setup_attempt(true)) { finish(); continue; }
             continue;
         }
-        // ---- corrector: improve g_{n-1} and b_0..b_{n-1}.  One code path for every substep
-        //      (the groups of a wave are at different substeps), tables from LDS. ----
+        // ---- corrector: improve g_{n-1} and b_0..b_{n-1}.  Straight-line code for every
+        //      substep (the groups of a wave are at different substeps): each row of b and g is
+        //      read from LDS at once and the substep-dependent steps are selected, not branched,
+        //      so the LDS latency is paid once per trip. ----
         double maxak = 0.0, maxdb6 = 0.0;
         {
             const double* rri = tab + kTabRRI + n * (n - 1) / 2;      // 1 / (h_n - h_j), j = 0..n-1
             const double* cc = tab + kTabC + (n - 1) * (n - 2) / 2;   // b_k += cc[k] dg_{n-1}, k < n-1
+            double r[7], w[6];
+#pragma unroll
+            for (int k = 0; k < 7; k++) r[k] = rri[k < n ? k : 0];
+#pragma unroll
+            for (int k = 0; k < 6; k++) w[k] = cc[k < n - 1 ? k : 0];
+            double rl = r[0];
+#pragma unroll
+            for (int k = 1; k < 7; k++) rl = (k == n - 1) ? r[k] : rl;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
+                double gv[7], bv[7];
+#pragma unroll
+                for (int k = 0; k < 7; k++) { gv[k] = g[k][c]; bv[k] = b[k][c]; }
                 // ((gk r_0 - g_0) r_1 - g_1) r_2 ..., each step one fused multiply-add
                 double val = at[c] - a0[c];
-                for (int k = 0; k < n - 1; k++) val = fma(val, rri[k], -g[k][c]);
-                val *= rri[n - 1];
-                const double tmp = val - g[n - 1][c];
+#pragma unroll
+                for (int k = 0; k < 6; k++) val = (k < n - 1) ? fma(val, r[k], -gv[k]) : val;
+                val *= rl;
+                double gold = gv[0];
+#pragma unroll
+                for (int k = 1; k < 7; k++) gold = (k == n - 1) ? gv[k] : gold;
+                const double tmp = val - gold;
                 g[n - 1][c] = val;
-                for (int k = 0; k < n - 1; k++) b[k][c] += tmp * cc[k];
-                b[n - 1][c] += tmp;
+#pragma unroll
+                for (int k = 0; k < 7; k++) {
+                    const double nb = (k < n - 1) ? fma(tmp, w[k < 6 ? k : 5], bv[k]) : ((k == n - 1) ? bv[k] + tmp : bv[k]);
+                    b[k][c] = nb;
+                }
                 if (n == 7 && own) {
                     const double ak = fabs(at[c]);
                     if (is_normal(ak) && ak > maxak) maxak = ak;
@@ -377,7 +397,7 @@ ias15_kernel(IntegrateArgs args) {
                 const double bk = fabs(b[6][c]);
                 if (is_normal(bk) && bk > mb6) mb6 = bk;
             }
-            const double err = gbcast<G>(mb6 / mak, 0);
+            const double err = gbcast<G, 0>(mb6 / mak);
             if (is_normal(err)) dt_new = pow(ka->io.epsilon / err, 1.0 / 7.0) * dts;
             else dt_new = dts / 0.25;
             if (fabs(dt_new) < ka->io.min_dt) dt_new = copysign(ka->io.min_dt, dt_new);
