@@ -188,11 +188,17 @@ AHD ChebOut<D> cheb_eval(const DevSlot& S, D t) {
     const D fi = (D(S.c_init) + t) * D(S.inv_intlen);
     int idx = (int)value_of(fi);
     idx = idx < 0 ? 0 : (idx >= S.nrec ? S.nrec - 1 : idx);
+#if defined(__HIP_DEVICE_COMPILE__)
+    // the records live in global memory: address them as such (global, not flat, loads)
+    typedef __attribute__((address_space(1))) const double GlobalDouble;
+    GlobalDouble* rec = (GlobalDouble*)S.rec + (size_t)idx * kRecWords;
+#else
     const double* rec = S.rec + (size_t)idx * kRecWords;
+#endif
     double c[kRecWords];
 #pragma unroll
     for (int k = 0; k < kRecWords; k += 2) {
-        const Pair v = *reinterpret_cast<const Pair*>(rec + k);
+        const Pair v = {rec[k], rec[k + 1]};
         c[k] = v.x;
         c[k + 1] = v.y;
     }
@@ -591,10 +597,11 @@ template <typename D, bool Mixed>
 struct TermScalar { using type = typename std::conditional<Mixed, typename SingleOf<D>::type, D>::type; };
 
 // Evaluates the force model for the real particle at (t, xr, vr) cooperatively over the G lanes
-// of a group; every lane receives the real acceleration and the Jacobian J.
+// of a group; every lane receives the real acceleration and the Jacobian J.  `slots` are the
+// ephemeris slot descriptors (E.slot, or a copy of it in LDS).
 template <int G, bool Mixed, typename D>
-AHD void force_eval(const DevEphem& E, const ForceOpts& O, int lane, D t, const D xr[3], const D vr[3],
-                    const double A[3], bool ng, bool need_jac, EvalResult<D>& out,
+AHD void force_eval(const DevEphem& E, const DevSlot* slots, const ForceOpts& O, int lane, D t, const D xr[3],
+                    const D vr[3], const double A[3], bool ng, bool need_jac, EvalResult<D>& out,
                     Jac<typename TermScalar<D, Mixed>::type>& J) {
     using TS = typename TermScalar<D, Mixed>::type;
     constexpr int NR = (kNumSlots + G - 1) / G;
@@ -614,10 +621,10 @@ AHD void force_eval(const DevEphem& E, const ForceOpts& O, int lane, D t, const 
         if (s < kNumSlots) {
             ChebOut<D> c;
             if (r == 0 && want_sun_vel) {
-                c = cheb_eval<D, true>(E.slot[s], t);
+                c = cheb_eval<D, true>(slots[s], t);
                 sunv_l[0] = c.vx; sunv_l[1] = c.vy; sunv_l[2] = c.vz;
             } else {
-                c = cheb_eval<D, false>(E.slot[s], t);
+                c = cheb_eval<D, false>(slots[s], t);
             }
             px[r][0] = c.x; px[r][1] = c.y; px[r][2] = c.z;
         } else {

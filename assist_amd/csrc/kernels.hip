@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) batch_forces_kernel(ForcesArgs a) {
     }
     EvalResult<double> R;
     Jac<TS> J;
-    force_eval<G, Mixed, double>(*a.E, a.fo, lane, a.t[sys], xr, vr, A, ng, nv > 1, R, J);
+    force_eval<G, Mixed, double>(*a.E, a.E->slot, a.fo, lane, a.t[sys], xr, vr, A, ng, nv > 1, R, J);
     // each lane handles vectors lane, lane+G, ... (G may be smaller than nvec for parity runs)
     for (int v = lane; v < nv; v += G) {
         double x[3], vv[3], out[3];
@@ -153,6 +153,14 @@ ias15_kernel(IntegrateArgs args) {
         else if (i < kTabC) v = cRRI[i - kTabRRI];
         else v = cC[i - kTabC];
         tab[i] = v;
+    }
+    // the ephemeris slot descriptors, read by every force evaluation with a lane-dependent
+    // index, are staged in LDS as well (a vector load of them would sit in front of every
+    // record fetch)
+    __shared__ DevSlot s_slots[kNumSlots];
+    {
+        const DevSlot* src = ka->E->slot;
+        for (int i = threadIdx.x; i < kNumSlots; i += blockDim.x) s_slots[i] = src[i];
     }
     __syncthreads();
     const int cols_per_group = ias15_cols_per_group(G, nv, W);
@@ -321,7 +329,7 @@ ias15_kernel(IntegrateArgs args) {
         EvalResult<double> R;
         Jac<TS> J;
         const ForceOpts fo = ka->fo;
-        force_eval<G, Mixed, double>(E, fo, lane, te, xr, vr, A, ng, nv > 1, R, J);
+        force_eval<G, Mixed, double>(E, s_slots, fo, lane, te, xr, vr, A, ng, nv > 1, R, J);
         double at[3];
         own_accel<double, TS>(lane, R, J, xe, ve, dA, ng, at);
         evals++;
