@@ -597,10 +597,10 @@ template <typename D, bool Mixed>
 struct TermScalar { using type = typename std::conditional<Mixed, typename SingleOf<D>::type, D>::type; };
 
 // Evaluates the force model for the real particle at (t, xr, vr) cooperatively over the G lanes
-// of a group; every lane receives the real acceleration and the Jacobian J.  `slots` are the
-// ephemeris slot descriptors (E.slot, or a copy of it in LDS).
+// of a group; every lane receives the real acceleration and the Jacobian J.  `slots` and `gm` are
+// the ephemeris slot descriptors and masses (E.slot and E.GM, or copies of them in LDS).
 template <int G, bool Mixed, typename D>
-AHD void force_eval(const DevEphem& E, const DevSlot* slots, const ForceOpts& O, int lane, D t, const D xr[3],
+AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, const ForceOpts& O, int lane, D t, const D xr[3],
                     const D vr[3], const double A[3], bool ng, bool need_jac, EvalResult<D>& out,
                     Jac<typename TermScalar<D, Mixed>::type>& J) {
     using TS = typename TermScalar<D, Mixed>::type;
@@ -669,7 +669,7 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const ForceOpts& O,
         if (s >= kNumSlots || s == kSlotEMB) continue;
         const int b = slot_body(s);
         if (b >= nbody) continue;
-        const D GM = D(E.GM[b]);
+        const D GM = D(gm[b]);
         const bool allowed = (b == 0) ? (forces & F_SUN) : (b < kNumPlanets ? (forces & F_PLANETS) : (forces & F_ASTEROIDS));
         const D dx = xr[0] - px[r][0], dy = xr[1] - px[r][1], dz = xr[2] - px[r][2];
         // Mixed form: an asteroid's point mass runs in FP32 unless the particle is within

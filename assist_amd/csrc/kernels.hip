@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) batch_forces_kernel(ForcesArgs a) {
     }
     EvalResult<double> R;
     Jac<TS> J;
-    force_eval<G, Mixed, double>(*a.E, a.E->slot, a.fo, lane, a.t[sys], xr, vr, A, ng, nv > 1, R, J);
+    force_eval<G, Mixed, double>(*a.E, a.E->slot, a.E->GM, a.fo, lane, a.t[sys], xr, vr, A, ng, nv > 1, R, J);
     // each lane handles vectors lane, lane+G, ... (G may be smaller than nvec for parity runs)
     for (int v = lane; v < nv; v += G) {
         double x[3], vv[3], out[3];
@@ -158,9 +158,11 @@ ias15_kernel(IntegrateArgs args) {
     // index, are staged in LDS as well (a vector load of them would sit in front of every
     // record fetch)
     __shared__ DevSlot s_slots[kNumSlots];
+    __shared__ double s_gm[kNumBodies];
     {
         const DevSlot* src = ka->E->slot;
         for (int i = threadIdx.x; i < kNumSlots; i += blockDim.x) s_slots[i] = src[i];
+        for (int i = threadIdx.x; i < kNumBodies; i += blockDim.x) s_gm[i] = ka->E->GM[i];
     }
     __syncthreads();
     const int cols_per_group = ias15_cols_per_group(G, nv, W);
@@ -329,7 +331,7 @@ ias15_kernel(IntegrateArgs args) {
         EvalResult<double> R;
         Jac<TS> J;
         const ForceOpts fo = ka->fo;
-        force_eval<G, Mixed, double>(E, s_slots, fo, lane, te, xr, vr, A, ng, nv > 1, R, J);
+        force_eval<G, Mixed, double>(E, s_slots, s_gm, fo, lane, te, xr, vr, A, ng, nv > 1, R, J);
         double at[3];
         own_accel<double, TS>(lane, R, J, xe, ve, dA, ng, at);
         evals++;

@@ -106,7 +106,7 @@ static Counts count_eval(const DevEphem& E, int forces, int nvec, bool ng) {
     g_ops64 = g_ops32 = 0;
     EvalResult<Cnt> R;
     Jac<TS> J;
-    force_eval<1, Mixed, Cnt>(E, E.slot, O, 0, Cnt(1.0), xr, vr, A, ng, nvec > 1, R, J);
+    force_eval<1, Mixed, Cnt>(E, E.slot, E.GM, O, 0, Cnt(1.0), xr, vr, A, ng, nvec > 1, R, J);
     for (int v = 1; v < nvec; v++) {
         Cnt out[3];
         const double dA[3] = {0, 0, 0};
