@@ -137,6 +137,47 @@ def integrate(ephem, t0, t1, states, params=None, extras=None, config: BatchConf
     return IntegrateResult(states, st, steps, stats)
 
 
+@dataclass
+class DenseResult:
+    outputs: list          # per system: (m_i, 1 + n_var, 6) states at its output times
+    states: np.ndarray     # end of each system's last step
+    status: np.ndarray
+    steps: np.ndarray
+    stats: BatchStats
+
+
+def integrate_dense(ephem, t0, out_times, states, params=None, extras=None,
+                    config: BatchConfig | None = None) -> DenseResult:
+    """Dense output (the batched counterpart of the reference's integrate_or_interpolate,
+    reference src/assist.c:642-680): system i is integrated from t0[i] through its output times
+    out_times[i] (sorted in one direction, not before t0[i]) and its state at each of them is
+    interpolated inside the IAS15 step that covers it."""
+    states = np.array(states, dtype=np.float64, copy=True, order="C")
+    n, nv, six = states.shape
+    assert six == 6 and len(out_times) == n
+    t0 = np.ascontiguousarray(np.broadcast_to(np.asarray(t0, dtype=np.float64), (n,)))
+    counts = np.array([len(np.atleast_1d(o)) for o in out_times], dtype=np.int64)
+    off = np.zeros(n + 1, dtype=np.int32)
+    off[1:] = np.cumsum(counts)
+    flat = np.ascontiguousarray(np.concatenate([np.atleast_1d(np.asarray(o, dtype=np.float64)) for o in out_times])
+                                if n else np.zeros(0))
+    out = np.zeros((max(int(off[-1]), 1), nv, 6))
+    st = np.zeros(n, dtype=np.int32)
+    steps = np.zeros(n, dtype=np.int64)
+    stats = BatchStats()
+    pp = None
+    if params is not None:
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        assert params.shape == (n, nv, 3)
+        pp = _d(params)
+    cfg = config or BatchConfig.default()
+    _check(clibassist.assist_batch_integrate_dense(
+        C.byref(ephem), _opts_ptr(extras), C.byref(cfg), n, nv - 1, _d(t0), _d(states), pp,
+        off.ctypes.data_as(_ip), _d(flat), _d(out), st.ctypes.data_as(_ip), steps.ctypes.data_as(_llp), C.byref(stats)))
+    outputs = [out[off[i]:off[i + 1]] for i in range(n)]
+    return DenseResult(outputs, states, st, steps, stats)
+
+
 class DeviceArray:
     """A device buffer owned by libassist (for inputs kept resident in HBM)."""
 

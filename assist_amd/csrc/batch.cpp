@@ -514,9 +514,12 @@ int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const str
                                      const double* d_t1, double* d_states, const double* d_params, int* d_status,
                                      long long* d_steps, long long* d_evals, void* stream,
                                      struct assist_batch_stats* stats, const struct assist_batch_extra* ex) {
-    if (n < 0 || n_var < 0 || n_var > 7 || !d_t0 || !d_t1 || !d_states || !d_status) return ASSIST_BATCH_ERROR_ARG;
+    const bool dense = ex && ex->out_t;
+    if (n < 0 || n_var < 0 || n_var > 7 || !d_t0 || (!d_t1 && !dense) || !d_states || !d_status)
+        return ASSIST_BATCH_ERROR_ARG;
     if (ex && ex->trace && (!ex->trace_count || ex->trace_capacity < 0 || ex->trace_every < 1 || ex->trace_skip < 0))
         return ASSIST_BATCH_ERROR_ARG;
+    if (dense && (!ex->out_offset || !ex->out_states)) return ASSIST_BATCH_ERROR_ARG;
     struct assist_batch_config cfg;
     if (cfg_in) cfg = *cfg_in;
     else assist_batch_config_default(&cfg);
@@ -550,6 +553,9 @@ int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const str
         a.trace_cap = ex->trace_capacity;
         a.trace_every = ex->trace_every;
         a.trace_skip = ex->trace_skip;
+        a.out_off = ex->out_offset;
+        a.out_t = ex->out_t;
+        a.out_states = ex->out_states;
     }
     hipStream_t s = (hipStream_t)stream;
     LaunchShape shape;
@@ -560,7 +566,7 @@ int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const str
     shape.waves = cfg.waves_per_simd > 0 ? cfg.waves_per_simd : 1;
     if (shape.waves != 1 && shape.waves != 2) return ASSIST_BATCH_ERROR_ARG;
     const bool tune = cfg.lanes_per_system == 0 && cfg.block_size == 0 && cfg.waves_per_simd == 0 &&
-                      n >= kTuneMinSystems;
+                      n >= kTuneMinSystems && !dense;
     if (tune) {
         rc = tuned_shape(a, mixed, s, &shape);
         if (rc) return rc;
@@ -650,7 +656,10 @@ void assist_additional_forces(struct reb_simulation* sim) {
     }
 }
 
-void reb_simulation_integrate(struct reb_simulation* sim, double tmax) {
+// Integrates every real particle of `sim` (with its variational particles) as its own system
+// to tmax; with `dense`, the steps are not shortened to land on tmax and the state at tmax is
+// interpolated inside the step that covers it.
+static void sim_integrate(struct reb_simulation* sim, double tmax, bool dense) {
     auto* ax = (struct assist_extras*)sim->extras;
     if (!ax || !ax->ephem) {
         reb_simulation_error(sim, "No ASSIST extras attached to the simulation.");
@@ -670,14 +679,26 @@ void reb_simulation_integrate(struct reb_simulation* sim, double tmax) {
     cfg.epsilon = sim->ri_ias15.epsilon;
     cfg.min_dt = sim->ri_ias15.min_dt;
     if (sim->dt != 0) cfg.dt_initial = fabs(sim->dt);
-    const int rc = assist_batch_integrate(ax->ephem, ax, &cfg, L.n_real, L.nvmax, t0.data(), t1.data(), states.data(),
-                                          hp ? params.data() : nullptr, st.data(), steps.data(), nullptr);
+    int rc;
+    std::vector<double> out;
+    if (dense) {
+        std::vector<int> off(L.n_real + 1);
+        for (int j = 0; j <= L.n_real; j++) off[j] = j;
+        out.assign((size_t)L.n_real * nv * 6, 0.0);
+        rc = assist_batch_integrate_dense(ax->ephem, ax, &cfg, L.n_real, L.nvmax, t0.data(), states.data(),
+                                          hp ? params.data() : nullptr, off.data(), t1.data(), out.data(), st.data(),
+                                          steps.data(), nullptr);
+    } else {
+        rc = assist_batch_integrate(ax->ephem, ax, &cfg, L.n_real, L.nvmax, t0.data(), t1.data(), states.data(),
+                                    hp ? params.data() : nullptr, st.data(), steps.data(), nullptr);
+    }
     if (rc) {
         reb_simulation_error(sim, assist_batch_error_string(rc));
         return;
     }
+    const std::vector<double>& res = dense ? out : states;
     for (int j = 0; j < L.n_real; j++) {
-        const double* s = &states[(size_t)j * nv * 6];
+        const double* s = &res[(size_t)j * nv * 6];
         reb_particle& p = sim->particles[j];
         p.x = s[0]; p.y = s[1]; p.z = s[2]; p.vx = s[3]; p.vy = s[4]; p.vz = s[5];
         for (size_t k = 0; k < L.vars[j].size(); k++) {
@@ -692,6 +713,64 @@ void reb_simulation_integrate(struct reb_simulation* sim, double tmax) {
         }
     }
     if (!sim->status) sim->t = tmax;
+}
+
+void reb_simulation_integrate(struct reb_simulation* sim, double tmax) { sim_integrate(sim, tmax, false); }
+
+void assist_integrate_or_interpolate(struct assist_extras* ax, double t) {
+    if (!ax || !ax->sim) return;
+    sim_integrate(ax->sim, t, true);
+}
+
+int assist_batch_integrate_dense(const struct assist_ephem* ephem, const struct assist_extras* opts,
+                                 const struct assist_batch_config* cfg, int n, int n_var, const double* t0,
+                                 double* states, const double* params, const int* out_offset, const double* out_t,
+                                 double* out_states, int* status, long long* steps, struct assist_batch_stats* stats) {
+    if (n < 0 || n_var < 0 || n_var > 7 || !t0 || !states || !out_offset || !out_t || !out_states)
+        return ASSIST_BATCH_ERROR_ARG;
+    if (n == 0) return ASSIST_SUCCESS;
+    if (out_offset[0] != 0) return ASSIST_BATCH_ERROR_ARG;
+    for (int i = 0; i < n; i++) {
+        const int a0 = out_offset[i], a1 = out_offset[i + 1];
+        if (a1 < a0) return ASSIST_BATCH_ERROR_ARG;
+        if (a1 == a0) continue;
+        const double dir = (out_t[a1 - 1] >= t0[i]) ? 1.0 : -1.0;
+        double prev = t0[i];
+        for (int j = a0; j < a1; j++) {
+            if (!((out_t[j] - prev) * dir >= 0.0)) return ASSIST_BATCH_ERROR_ARG;
+            prev = out_t[j];
+        }
+    }
+    if (assist_gpu_device_count() <= 0) return ASSIST_BATCH_ERROR_DEVICE;
+    const int nv = 1 + n_var;
+    const size_t ns = (size_t)n, m = (size_t)out_offset[n];
+    DevBuf<double> d0, ds, dp, dt, dout;
+    DevBuf<int> doff, dst;
+    DevBuf<long long> dsteps;
+    if (!d0.alloc(ns) || !ds.alloc(ns * nv * 6) || !dst.alloc(ns) || !dsteps.alloc(ns) || !doff.alloc(ns + 1) ||
+        !dt.alloc(m ? m : 1) || !dout.alloc(m ? m * nv * 6 : 1) || (params && !dp.alloc(ns * nv * 3)))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    if (!hip_ok(hipMemcpy(d0.p, t0, ns * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy") ||
+        !hip_ok(hipMemcpy(ds.p, states, ns * nv * 6 * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy") ||
+        !hip_ok(hipMemcpy(doff.p, out_offset, (ns + 1) * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy") ||
+        (m && !hip_ok(hipMemcpy(dt.p, out_t, m * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")) ||
+        (params && !hip_ok(hipMemcpy(dp.p, params, ns * nv * 3 * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    struct assist_batch_extra ex;
+    memset(&ex, 0, sizeof ex);
+    ex.trace_every = 1;
+    ex.out_offset = doff.p;
+    ex.out_t = dt.p;
+    ex.out_states = dout.p;
+    const int rc = assist_batch_integrate_device_ex(ephem, opts, cfg, n, n_var, d0.p, nullptr, ds.p, params ? dp.p : nullptr,
+                                                    dst.p, dsteps.p, nullptr, nullptr, stats, &ex);
+    if (rc) return rc;
+    if (!hip_ok(hipMemcpy(states, ds.p, ns * nv * 6 * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy") ||
+        (m && !hip_ok(hipMemcpy(out_states, dout.p, m * nv * 6 * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy")) ||
+        (status && !hip_ok(hipMemcpy(status, dst.p, ns * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy")) ||
+        (steps && !hip_ok(hipMemcpy(steps, dsteps.p, ns * sizeof(long long), hipMemcpyDeviceToHost), "hipMemcpy")))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    return ASSIST_SUCCESS;
 }
 
 int assist_step_features(const struct assist_ephem* ephem, int n, const double* d_rows, int stride, float* d_X,

@@ -59,6 +59,12 @@ struct IntegrateArgs {
     int trace_every;        // record steps s >= trace_skip with s % trace_every == 0
     int trace_skip;
     int pad_;
+    // dense output (optional): system s writes its state at out_t[out_off[s] .. out_off[s+1]-1]
+    // (sorted in the direction of integration) to out_states[i][nvec][6] by interpolating the
+    // accepted step that contains each time; steps are then not trimmed to land on t1 (t1 unused)
+    const int* out_off;
+    const double* out_t;
+    double* out_states;
 };
 
 // Trace record: t, x0[3], v0[3] at the start of the step, the previous step, the accepted

@@ -176,6 +176,7 @@ ias15_kernel(IntegrateArgs args) {
     bool ng = false;
     int sys = grp;
     bool active = sys < ka->nsys;
+    int cur = 0, cend = 0;  // dense output: next and end index of this system's output times
 
     auto load = [&](int s) {
         const double* st = ka->states + ((size_t)s * nv + (own ? lane : 0)) * 6;
@@ -198,7 +199,13 @@ ias15_kernel(IntegrateArgs args) {
             if (own) { dA[0] = pp[3 * lane]; dA[1] = pp[3 * lane + 1]; dA[2] = pp[3 * lane + 2]; }
         }
         t = ka->t0[s];
-        t_end = ka->t1[s];
+        if (ka->out_t) {
+            cur = ka->out_off[s];
+            cend = ka->out_off[s + 1];
+            t_end = (cend > cur) ? ka->out_t[cend - 1] : t;
+        } else {
+            t_end = ka->t1[s];
+        }
         dt = ka->dt0 ? fabs(ka->dt0[s]) : ka->io.dt_initial;
         if (dt * (t_end - t) < 0) dt = -dt;
         dt_last = 0.0;
@@ -234,7 +241,7 @@ ias15_kernel(IntegrateArgs args) {
         if (new_step) {
             dts = dt;
             last = 0;
-            if ((t + dts - t_end) * dir >= 0) { dts = t_end - t; last = 1; }
+            if (!ka->out_t && (t + dts - t_end) * dir >= 0) { dts = t_end - t; last = 1; }
             attempts = 0;
         }
         if (!(fabs(dts) > 0.0) || !(fabs(dts) < 1e300) || ++attempts > 64) { status = kDtUnderflow; return false; }
@@ -295,7 +302,8 @@ ias15_kernel(IntegrateArgs args) {
     while (active) {
         ka = launder(ka);
         const DevEphem& E = *ka->E;
-        if (phase == 0 && t == t_end) { finish(); continue; }  // zero-length span
+        // zero-length span (dense output: no output times left)
+        if (phase == 0 && (ka->out_t ? cur >= cend : t == t_end)) { finish(); continue; }
         // ---- evaluation point of this trip ----
         double xe[3], ve[3], te;
         if (phase == 0) {
@@ -438,6 +446,44 @@ ias15_kernel(IntegrateArgs args) {
                     for (int c = 0; c < 3; c++) rec[13 + 3 * k + c] = b[k][c];
             }
         }
+        if (ka->out_t) {
+            // dense output: every output time in [t, t + dts) from this step's polynomial, as
+            // reference src/assist.c:556-597 (assist_interpolate)
+            while (cur < cend) {
+                double h = (ka->out_t[cur] - t) / dts;
+                if (!(h < 1.0)) break;
+                if (h < 0.0) h = 0.0;
+                if (own) {
+                    double s[9], sv[8];
+                    s[0] = dts * h;
+                    s[1] = s[0] * s[0] / 2.;
+                    s[2] = s[1] * h / 3.;
+                    s[3] = s[2] * h / 2.;
+                    s[4] = 3. * s[3] * h / 5.;
+                    s[5] = 2. * s[4] * h / 3.;
+                    s[6] = 5. * s[5] * h / 7.;
+                    s[7] = 3. * s[6] * h / 4.;
+                    s[8] = 7. * s[7] * h / 9.;
+                    sv[0] = dts * h;
+                    sv[1] = sv[0] * h / 2.;
+                    sv[2] = 2. * sv[1] * h / 3.;
+                    sv[3] = 3. * sv[2] * h / 4.;
+                    sv[4] = 4. * sv[3] * h / 5.;
+                    sv[5] = 5. * sv[4] * h / 6.;
+                    sv[6] = 6. * sv[5] * h / 7.;
+                    sv[7] = 7. * sv[6] * h / 8.;
+                    double* o = ka->out_states + ((size_t)cur * nv + lane) * 6;
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        o[c] = x0[c] + (s[8] * b[6][c] + s[7] * b[5][c] + s[6] * b[4][c] + s[5] * b[3][c] + s[4] * b[2][c] +
+                                        s[3] * b[1][c] + s[2] * b[0][c] + s[1] * a0[c] + s[0] * v0[c]);
+                        o[3 + c] = v0[c] + sv[7] * b[6][c] + sv[6] * b[5][c] + sv[5] * b[4][c] + sv[4] * b[3][c] +
+                                   sv[3] * b[2][c] + sv[2] * b[1][c] + sv[1] * b[0][c] + sv[0] * a0[c];
+                    }
+                }
+                cur++;
+            }
+        }
         // end-of-step weights: the predictor at h = 1 (row 8 of the tables)
         double wx[8], wv[7];
         {
@@ -491,7 +537,7 @@ ias15_kernel(IntegrateArgs args) {
         for (int c = 0; c < 3; c++) bad |= !(fabs(x0[c]) < 1e300) || !(fabs(v0[c]) < 1e300);
         bad = gor<G>(own ? bad : 0);
         if (bad) { status = kNonFinite; finish(); continue; }
-        if (last) { finish(); continue; }
+        if (last || (ka->out_t && cur >= cend)) { finish(); continue; }
         if (steps >= ka->io.max_steps) { status = kMaxSteps; finish(); continue; }
         phase = 0;
     }

@@ -41,6 +41,11 @@ class Extras(Structure):
             clibassist.assist_free_pointers(byref(self))
             self._initialised = False
 
+    def integrate_or_interpolate(self, t: float) -> None:
+        """Bring the simulation to time t (reference assist/extras.py:47-48): each real particle is
+        integrated as its own system and its state at t interpolated inside the covering step."""
+        clibassist.assist_integrate_or_interpolate(byref(self), c_double(t))
+
     def detach(self, sim: Simulation) -> None:
         sim._extras_ref = None
         clibassist.assist_detach(byref(sim), byref(self))

@@ -300,6 +300,15 @@ struct assist_batch_extra {
     int trace_every;          /* record accepted steps s >= trace_skip with s % trace_every == 0 */
     int trace_skip;
     int reserved_;
+    /* Dense output (all NULL for none): system i writes its state (every vector) at the times
+     * out_t[out_offset[i] .. out_offset[i+1]-1], sorted in the direction of integration and not
+     * before t0, to out_states[j][1 + n_var][6], interpolating the accepted step that contains each
+     * time with its IAS15 polynomial (reference src/assist.c:556-597, 642-680).  Steps are then
+     * not shortened to land on t1 (d_t1 may be NULL); the system stops after the step that covers
+     * its last output time, and `states` receives the state at the end of that step. */
+    const int* out_offset;
+    const double* out_t;
+    double* out_states;
 };
 
 /* Same as assist_batch_integrate_device, with the learned start and step traces of `ex`. */
@@ -308,6 +317,20 @@ int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const str
                                      const double* d_t1, double* d_states, const double* d_params, int* d_status,
                                      long long* d_steps, long long* d_evals, void* stream,
                                      struct assist_batch_stats* stats, const struct assist_batch_extra* ex);
+
+/* Dense output with host buffers: out_offset[n + 1], out_t[out_offset[n]], out_states
+ * [out_offset[n]][1 + n_var][6]; the output times of each system are checked to be sorted in one
+ * direction from t0[i] (ASSIST_BATCH_ERROR_ARG otherwise).  states receive the end of each
+ * system's last step. */
+int assist_batch_integrate_dense(const struct assist_ephem* ephem, const struct assist_extras* opts,
+                                 const struct assist_batch_config* cfg, int n, int n_var, const double* t0,
+                                 double* states, const double* params, const int* out_offset, const double* out_t,
+                                 double* out_states, int* status, long long* steps, struct assist_batch_stats* stats);
+
+/* The reference's dense-output call (reference src/assist.c:642-680): brings the simulation to
+ * time t by integrating every real particle (with its variational particles) as its own system
+ * and interpolating inside the step that covers t; sim->t becomes t. */
+void assist_integrate_or_interpolate(struct assist_extras* ax, double t);
 
 /* Device buffer helpers for callers without their own allocator. */
 void* assist_device_malloc(size_t bytes);

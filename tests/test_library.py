@@ -183,3 +183,27 @@ def test_elements_round_trip(model):
     np.testing.assert_allclose(i2, inc, atol=1e-12)
     d = lambda x, y: np.abs((x - y + np.pi) % (2 * np.pi) - np.pi)
     assert d(n2, node).max() < 1e-10 and d(w2 + M2, argp + M).max() < 1e-9
+
+
+def test_dense_output_checks_time_order_before_the_gpu(ephem):
+    """assist_batch_integrate_dense validates each system's output times (sorted in one
+    direction from t0) on the host, before any device work."""
+    import ctypes as C
+    import assist_amd
+    lib = assist_amd.clibassist
+    from assist_amd import batch
+    states = np.zeros((1, 1, 6))
+    states[0, 0, 0] = 2.0
+    t0 = np.zeros(1)
+    off = np.array([0, 2], dtype=np.int32)
+    out = np.zeros((2, 1, 6))
+    st = np.zeros(1, dtype=np.int32)
+    steps = np.zeros(1, dtype=np.int64)
+    dp = C.POINTER(C.c_double)
+    for times in (np.array([10.0, 5.0]), np.array([-1.0, 5.0])):
+        rc = lib.assist_batch_integrate_dense(C.byref(ephem), None, None, 1, 0, t0.ctypes.data_as(dp),
+                                              states.ctypes.data_as(dp), None, off.ctypes.data_as(C.POINTER(C.c_int)),
+                                              times.ctypes.data_as(dp), out.ctypes.data_as(dp),
+                                              st.ctypes.data_as(C.POINTER(C.c_int)),
+                                              steps.ctypes.data_as(C.POINTER(C.c_longlong)), None)
+        assert rc == 100
