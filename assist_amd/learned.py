@@ -57,9 +57,12 @@ clibassist.assist_batch_predict_start.argtypes = [C.c_void_p, C.c_void_p, C.c_in
 
 
 class BatchExtra(C.Structure):
+    """ctypes mirror of ``struct assist_batch_extra`` (every field, so that the library never
+    reads past the end of a Python-built struct; the dense-output pointers stay NULL here)."""
     _fields_ = [("dt0", C.c_void_p), ("bw", C.c_void_p), ("trace", C.c_void_p), ("trace_count", C.c_void_p),
                 ("trace_capacity", C.c_int), ("trace_every", C.c_int), ("trace_skip", C.c_int),
-                ("reserved_", C.c_int)]
+                ("reserved_", C.c_int), ("out_offset", C.c_void_p), ("out_t", C.c_void_p),
+                ("out_states", C.c_void_p)]
 
 
 def _p(d):

@@ -146,6 +146,45 @@ def test_struct_layouts_match_reference_offsets():
     assert Extras.alpha.offset == 72 and Extras.r0.offset == 104
 
 
+def _c_layout(struct_name, fields):
+    """(sizeof, {field: offsetof}) of a struct of include/assist.h, from gcc."""
+    import json
+    import subprocess
+    src = "#include <stdio.h>\n#include <stddef.h>\n#include \"assist.h\"\nint main(void){printf(\"{\\\"size\\\": %zu\", sizeof(struct " + struct_name + "));"
+    for f in fields:
+        src += "printf(\", \\\"%s\\\": %%zu\", offsetof(struct %s, %s));" % (f, struct_name, f)
+    src += "printf(\"}\\n\"); return 0;}\n"
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "layout")
+        subprocess.run(["gcc", "-x", "c", "-I", os.path.join(ROOT, "include"), "-o", exe, "-"], input=src.encode(),
+                       check=True)
+        out = json.loads(subprocess.run([exe], check=True, capture_output=True).stdout)
+    return out.pop("size"), out
+
+
+def test_struct_mirrors_match_the_header():
+    """Every ctypes mirror of a struct of include/assist.h has the C struct's fields at the C
+    offsets and the C size (a short mirror makes the library read or write past the end of the
+    caller's struct).  Mirror fields named with a leading underscore wrap the C field."""
+    import assist_amd
+    from assist_amd.batch import BatchConfig, BatchStats
+    from assist_amd.extras import Extras
+    from assist_amd.learned import BatchExtra
+    from assist_amd.simulation import IntegratorIAS15, Particle, Simulation, VariationalConfiguration
+    mirrors = ((BatchConfig, "assist_batch_config"), (BatchStats, "assist_batch_stats"),
+               (BatchExtra, "assist_batch_extra"), (assist_amd.Ephem, "assist_ephem"), (Extras, "assist_extras"),
+               (Particle, "reb_particle"), (Simulation, "reb_simulation"),
+               (VariationalConfiguration, "reb_variational_configuration"),
+               (IntegratorIAS15, "reb_simulation_integrator_ias15"))
+    for cls, name in mirrors:
+        fields = [f[0] for f in cls._fields_]
+        cnames = [f.lstrip("_") if f != "reserved_" else f for f in fields]
+        size, offs = _c_layout(name, cnames)
+        assert C.sizeof(cls) == size, name
+        for f, c in zip(fields, cnames):
+            assert getattr(cls, f).offset == offs[c], (name, f)
+
+
 def test_gpu_entry_points_fail_without_gpu(ephem):
     from assist_amd import batch
     if batch.gpu_count() > 0:
