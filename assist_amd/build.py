@@ -22,7 +22,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("ASSIST_AMD_ARCH", "gfx950")
 
 HIP_SOURCES = ["kernels.hip", "mlp.hip"]
-CPP_SOURCES = ["spk.cpp", "assist_api.cpp", "batch.cpp"]
+CPP_SOURCES = ["spk.cpp", "ascii.cpp", "assist_api.cpp", "batch.cpp"]
 HEADERS = ["device_types.h", "force_model.h", "ias15_tables.h", "kernels.h", "internal.h"]
 
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
@@ -42,7 +42,7 @@ def _run(cmd):
 
 def sources():
     return [os.path.join(CSRC, f) for f in HIP_SOURCES + CPP_SOURCES + HEADERS] + [
-        os.path.join(ROOT, "include", "assist.h")]
+        os.path.join(ROOT, "include", "assist.h"), os.path.join(ROOT, "include", "ascii_ephem.h")]
 
 
 def up_to_date():

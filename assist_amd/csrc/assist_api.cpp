@@ -12,6 +12,7 @@
 
 #include <vector>
 
+#include "../../include/ascii_ephem.h"
 #include "internal.h"
 
 #define ASSIST_STR2(s) #s
@@ -93,6 +94,13 @@ static int planets_calc_cb(const struct assist_ephem* e, double jd_ref, double j
     return assist_host::planets_calc(e, jd_ref, jd_rel, body, GM, x, y, z, vx, vy, vz, ax, ay, az);
 }
 
+static int ascii_planets_calc_cb(const struct assist_ephem* e, double jd_ref, double jd_rel, int body, double* const GM,
+                                 double* const x, double* const y, double* const z, double* const vx,
+                                 double* const vy, double* const vz, double* const ax, double* const ay,
+                                 double* const az) {
+    return assist_ascii_calc_from_ephem(e, jd_ref, jd_rel, body, GM, x, y, z, vx, vy, vz, ax, ay, az);
+}
+
 int assist_ephem_init(struct assist_ephem* ephem, char* user_planets_path, char* user_asteroids_path) {
     const int FN = 1024;
     char planets_path[FN], asteroids_path[FN];
@@ -123,11 +131,55 @@ int assist_ephem_init(struct assist_ephem* ephem, char* user_planets_path, char*
         snprintf(asteroids_path, FN, "%s", user_asteroids_path);
     }
     if (!ast_missing && !(ephem->spk_asteroids = assist_host::spk_init(asteroids_path))) ast_missing = 1;
+    if (fmt == FILE_FORMAT_ASCII_BIN) {
+        // planets from a JPL binary file: its constants table gives the force-model constants and
+        // the asteroid masses (reference src/assist.c:291-327)
+        ephem->ascii_planets = assist_ascii_init(planets_path);
+        if (!ephem->ascii_planets) {
+            if (ephem->spk_asteroids) { assist_host::spk_free(ephem->spk_asteroids); ephem->spk_asteroids = nullptr; }
+            return ASSIST_ERROR_EPHEM_FILE;
+        }
+        const ascii_s* a = ephem->ascii_planets;
+        ephem->planets_source = FILE_FORMAT_ASCII_BIN;
+        for (int k = 0; k < ASSIST_BODY_NPLANETS; k++) ephem->spk_target_index[k] = -1;
+        ephem->spk_emb_index = -1;
+        ephem->J2E = a->J2E;
+        ephem->J3E = a->J3E;
+        ephem->J4E = a->J4E;
+        ephem->J2SUN = a->J2SUN;
+        ephem->AU = a->AU;
+        ephem->RE = a->RE;
+        ephem->CLIGHT = a->CLIGHT;
+        ephem->ASUN = a->ASUN;
+        ephem->EMRAT = a->cem;
+        ephem->Re_eq = ephem->RE / ephem->AU;
+        ephem->Rs_eq = ephem->ASUN / ephem->AU;
+        ephem->c_AU_per_day = (ephem->CLIGHT / ephem->AU) * 86400.0;
+        ephem->c_squared = ephem->c_AU_per_day * ephem->c_AU_per_day;
+        ephem->over_c_squared = 1.0 / ephem->c_squared;
+        ephem->planets_calc = ascii_planets_calc_cb;
+        if (ephem->spk_asteroids) {
+            for (auto& t : ephem->spk_asteroids->targets) {
+                if (t.code >= 2000000) {
+                    char key[16];  // only the first 6 characters take part in the match
+                    snprintf(key, sizeof key, "MA%04d", t.code - 2000000);
+                    double gm = 0.0;
+                    if (assist_ascii_find_constant(a, key, &gm)) t.mass = gm;
+                } else if (t.code == 399) {
+                    t.mass = a->mass[ASSIST_BODY_EARTH];
+                } else if (t.code == 301) {
+                    t.mass = a->mass[ASSIST_BODY_MOON];
+                } else if (t.code == 10) {
+                    t.mass = a->mass[ASSIST_BODY_SUN];
+                }
+            }
+        }
+        if (ast_missing) fprintf(stderr, "(ASSIST) %s\n", assist_error_messages[ASSIST_ERROR_AST_FILE]);
+        return ASSIST_SUCCESS;
+    }
     if (fmt != FILE_FORMAT_VALID_BSP) {
-        if (fmt == FILE_FORMAT_ASCII_BIN)
-            fprintf(stderr, "(ASSIST) Error: JPL .440/.441 planet files are not supported by this build.\n");
-        else
-            fprintf(stderr, "(ASSIST) Error: Failed to initialize planets ephemeris from '%s'.\n", planets_path);
+        fprintf(stderr, "(ASSIST) Error: Failed to initialize planets ephemeris from '%s'.\n", planets_path);
+        fprintf(stderr, "(ASSIST) Supported planet files: NAIF SPK kernels (.bsp) and JPL binary ephemerides (.440/.441).\n");
         if (ephem->spk_asteroids) { assist_host::spk_free(ephem->spk_asteroids); ephem->spk_asteroids = nullptr; }
         return ASSIST_ERROR_EPHEM_FILE;
     }
@@ -178,8 +230,10 @@ void assist_ephem_free_pointers(struct assist_ephem* ephem) {
     if (!ephem) return;
     if (ephem->spk_planets) assist_host::spk_free(ephem->spk_planets);
     if (ephem->spk_asteroids) assist_host::spk_free(ephem->spk_asteroids);
+    if (ephem->ascii_planets) assist_ascii_free(ephem->ascii_planets);
     ephem->spk_planets = nullptr;
     ephem->spk_asteroids = nullptr;
+    ephem->ascii_planets = nullptr;
 }
 
 void assist_ephem_free(struct assist_ephem* ephem) {
@@ -190,6 +244,10 @@ void assist_ephem_free(struct assist_ephem* ephem) {
 void assist_ephem_time_bounds(const struct assist_ephem* ephem, double* t_beg, double* t_end) {
     if (!ephem) return;
     double beg = -INFINITY, end = INFINITY;
+    if (!ephem->spk_planets && ephem->ascii_planets) {
+        beg = ephem->ascii_planets->beg;
+        end = ephem->ascii_planets->end;
+    }
     const spk_s* files[2] = {ephem->spk_planets, ephem->spk_asteroids};
     for (const spk_s* s : files) {
         if (!s) continue;
@@ -219,7 +277,9 @@ int assist_all_ephem(const struct assist_ephem* ephem, struct assist_ephem_cache
     }
     const double jd_ref = ephem->jd_ref;
     if (i < ASSIST_BODY_NPLANETS) {
-        const int f = assist_host::planets_calc(ephem, jd_ref, t, i, GM, x, y, z, vx, vy, vz, ax, ay, az);
+        const int f = ephem->planets_calc
+                          ? ephem->planets_calc(ephem, jd_ref, t, i, GM, x, y, z, vx, vy, vz, ax, ay, az)
+                          : assist_host::planets_calc(ephem, jd_ref, t, i, GM, x, y, z, vx, vy, vz, ax, ay, az);
         if (f != ASSIST_SUCCESS) return f;
     } else {
         double p[3];

@@ -11,6 +11,7 @@
 #include <tuple>
 #include <vector>
 
+#include "../../include/ascii_ephem.h"
 #include "device_types.h"
 #include "internal.h"
 #include "kernels.h"
@@ -98,22 +99,69 @@ bool flatten_target(const spk_s* s, const spk_target& t, double jd_ref, std::vec
     return true;
 }
 
+// Re-lays one column of a JPL binary planets file as device records, one per sub-interval of
+// `inc / niv` days: [jd_ref - JD(mid), 1/radius (1/day), x, y, z coefficients * scale], each
+// coordinate zero-padded to kCoefPad.  The evaluation is then the SPK one: the record index
+// (jd_ref - beg + t) niv / inc and z = (jd_ref - mid + t) / radius are the block and sub-interval
+// selection of reference src/ascii_ephem.c:315-318 and :34-39, written over the whole file.
+// `scale` folds a constant factor into the coefficients (the Earth's and the Moon's offsets from
+// the EMB are -1/(1+EMRAT) and EMRAT/(1+EMRAT) times the geocentric Moon).
+bool flatten_ascii_column(const ascii_s* a, int col, double scale, double jd_ref, std::vector<double>& out,
+                          DevSlot& slot, size_t& offset_words) {
+    const int ncf = a->ncf[col], niv = a->niv[col];
+    if (a->ncm[col] != 3 || ncf < 2 || ncf > kCoefPad || niv < 1) return false;
+    const size_t nblk = (size_t)ceil((a->end - a->beg) / a->inc);
+    const size_t blk_words = a->rec / sizeof(double);
+    if (nblk < 1 || (nblk + 2) * a->rec > a->len) return false;
+    const double sub = a->inc / niv;
+    const double* base = (const double*)a->map;
+    offset_words = out.size();
+    out.resize(out.size() + nblk * niv * kRecWords, 0.0);
+    double* dst = out.data() + offset_words;
+    for (size_t blk = 0; blk < nblk; blk++) {
+        const double* block = base + (blk + 2) * blk_words + a->off[col];
+        for (int k = 0; k < niv; k++) {
+            const double* cf = block + (size_t)ncf * 3 * k;
+            dst[0] = jd_ref - (a->beg + (double)blk * a->inc + (k + 0.5) * sub);
+            dst[1] = 2.0 / sub;
+            for (int m = 0; m < 3; m++)
+                for (int p = 0; p < ncf; p++) dst[2 + m * kCoefPad + p] = cf[m * ncf + p] * scale;
+            dst += kRecWords;
+        }
+    }
+    slot.rec = nullptr;
+    slot.c_init = jd_ref - a->beg;
+    slot.inv_intlen = 1.0 / sub;
+    slot.nrec = (int)(nblk * niv);
+    slot.P = ncf;
+    return true;
+}
+
 // Builds (or reuses) the device copy of the ephemeris on the current device.
 int get_device_ephem(const assist_ephem* e, const DevEphem** out, DevEphem* host_copy) {
     std::lock_guard<std::mutex> lock(g_mutex);
-    if (!e || !e->spk_planets) return ASSIST_ERROR_EPHEM_FILE;
-    if (e->ascii_planets) return ASSIST_BATCH_ERROR_UNSUPPORTED;
+    if (!e || (!e->spk_planets && !e->ascii_planets)) return ASSIST_ERROR_EPHEM_FILE;
     int dev = 0;
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return ASSIST_BATCH_ERROR_DEVICE;
     spk_s* pl = e->spk_planets;
-    DeviceEphemCache* c = pl->gpu;
+    ascii_s* jpl = pl ? nullptr : e->ascii_planets;
+    DeviceEphemCache* c = pl ? pl->gpu : (DeviceEphemCache*)jpl->device_cache;
     if (c && c->device == dev && c->jd_ref == e->jd_ref && c->asteroids == e->spk_asteroids) {
         *out = c->d_ephem;
         if (host_copy) *host_copy = c->host;
         return ASSIST_SUCCESS;
     }
-    if (c) { free_device_ephem(c); pl->gpu = nullptr; }
+    if (c) {
+        free_device_ephem(c);
+        if (pl) pl->gpu = nullptr;
+        else jpl->device_cache = nullptr;
+    }
     static const int planet_codes[12] = {10, 1, 2, 3, 399, 301, 4, 5, 6, 7, 8, 9};
+    // slot -> JPL binary column, and the factor folded into its coefficients
+    const int jpl_col[12] = {ASCII_SUN, ASCII_MER, ASCII_VEN, ASCII_EMB, ASCII_LUN, ASCII_LUN,
+                             ASCII_MAR, ASCII_JUP, ASCII_SAT, ASCII_URA, ASCII_NEP, ASCII_PLU};
+    const double cem = jpl ? jpl->cem : 0.0;
+    const double jpl_scale[12] = {1, 1, 1, 1, -1.0 / (1.0 + cem), cem / (1.0 + cem), 1, 1, 1, 1, 1, 1};
     DevEphem H;
     memset(&H, 0, sizeof H);
     std::vector<double> words;
@@ -121,6 +169,12 @@ int get_device_ephem(const assist_ephem* e, const DevEphem** out, DevEphem* host
     const int n_ast = e->spk_asteroids ? e->spk_asteroids->num : 0;
     if (n_ast > 16) return ASSIST_BATCH_ERROR_UNSUPPORTED;
     for (int s = 0; s < kNumSlots; s++) {
+        if (s < kFirstAsteroidSlot && jpl) {
+            if (!flatten_ascii_column(jpl, jpl_col[s], jpl_scale[s], e->jd_ref, words, H.slot[s], offs[s]))
+                return ASSIST_BATCH_ERROR_UNSUPPORTED;
+            H.slot[s].kind = 0;
+            continue;
+        }
         const spk_s* file = s < kFirstAsteroidSlot ? pl : e->spk_asteroids;
         int idx;
         if (s < kFirstAsteroidSlot) idx = assist_host::find_target(pl, planet_codes[s]);
@@ -139,15 +193,21 @@ int get_device_ephem(const assist_ephem* e, const DevEphem** out, DevEphem* host
     static const int body_slot[kNumPlanets] = {0, 1, 2, 4, 5, 6, 7, 8, 9, 10, 11};
     for (int b = 0; b < kNumBodies; b++) {
         if (b < kNumPlanets) {
-            const int idx = assist_host::find_target(pl, planet_codes[body_slot[b]]);
-            H.GM[b] = idx >= 0 ? pl->targets[idx].mass : 0.0;
+            if (jpl) {
+                H.GM[b] = jpl->mass[b];
+            } else {
+                const int idx = assist_host::find_target(pl, planet_codes[body_slot[b]]);
+                H.GM[b] = idx >= 0 ? pl->targets[idx].mass : 0.0;
+            }
         } else {
             H.GM[b] = (b - kNumPlanets) < n_ast ? e->spk_asteroids->targets[b - kNumPlanets].mass : 0.0;
         }
     }
-    H.inv_au = 1.0 / e->AU;
+    // planets: km -> au with the file's own au (reference src/spk.c:590, src/ascii_ephem.c:369)
+    const double au_km = jpl ? jpl->cau : e->AU;
+    H.inv_au = 1.0 / au_km;
     H.inv_au_ast = 1.0 / 149597870.7;
-    H.vel_scale = 86400.0 / e->AU;
+    H.vel_scale = 86400.0 / au_km;
     assist_ephem_time_bounds(e, &H.t_beg, &H.t_end);
     H.n_ast = n_ast;
     H.J2E = e->J2E; H.J3E = e->J3E; H.J4E = e->J4E; H.J2SUN = e->J2SUN;
@@ -175,7 +235,8 @@ int get_device_ephem(const assist_ephem* e, const DevEphem** out, DevEphem* host
         return ASSIST_BATCH_ERROR_DEVICE;
     }
     c->host = H;
-    pl->gpu = c;
+    if (pl) pl->gpu = c;
+    else jpl->device_cache = c;
     *out = c->d_ephem;
     if (host_copy) *host_copy = H;
     return ASSIST_SUCCESS;

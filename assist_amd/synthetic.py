@@ -434,9 +434,142 @@ def write_synthetic_ephemeris(planets_path, asteroids_path, seed=0, planet_mass_
     return model
 
 
+# --------------------------------------------------------------------------------------
+# JPL binary (.440/.441) planets file
+# --------------------------------------------------------------------------------------
+
+# Columns of a JPL binary ephemeris block, in file order, with DE440's coefficient counts and
+# sub-intervals per 32-day block: (name, coefficients, sub-intervals, components).  The reader's
+# layout is reference src/ascii_ephem.c:120-185 (header at 0x0A5C, names at 0x00FC and 0x0B28,
+# constants in block 1, data from block 2) and src/ascii_ephem.h:42-61 (column order).
+ASCII_COLUMNS = [
+    ("MER", 14, 4, 3), ("VEN", 10, 2, 3), ("EMB", 13, 2, 3), ("MAR", 11, 1, 3), ("JUP", 8, 1, 3),
+    ("SAT", 7, 1, 3), ("URA", 6, 1, 3), ("NEP", 6, 1, 3), ("PLU", 6, 1, 3), ("LUN", 13, 8, 3),
+    ("SUN", 11, 2, 3), ("NUT", 10, 4, 2), ("LIB", 10, 4, 3), ("MAN", 0, 0, 3), ("TDB", 0, 0, 1),
+]
+ASCII_BLOCK_DAYS = 32.0
+ASCII_TITLE = ["ASSIST-AMD synthetic ephemeris, JPL binary layout (Kepler orbits, not physical)",
+               "Start Epoch: JED=  %.1f" % (JD_J2000 - COVERAGE_DAYS),
+               "Final Epoch: JED=  %.1f" % (JD_J2000 + COVERAGE_DAYS)]
+ASCII_FILLER_CONSTANTS = 400  # names past this index go to the second name table
+
+
+def ascii_column_offsets():
+    """1-based word offsets of the columns inside a block (the first two words hold the
+    block's start and end JD)."""
+    offs, o = [], 3
+    for _, ncf, niv, ncm in ASCII_COLUMNS:
+        offs.append(o)
+        o += ncf * niv * ncm
+    return offs, o - 1  # offsets, words per block
+
+
+def _ascii_constants(model, constants=None):
+    c = dict(CONSTANTS, **(constants or {}))
+    names = [("DENUM", 440.0), ("LENUM", 440.0), ("TDATEF", 0.0), ("TDATEB", 0.0), ("JDEPOC", 2440400.5),
+             ("CENTER", 0.0), ("CLIGHT", c["CLIGHT"]), ("BETA", 1.0), ("GAMMA", 1.0), ("AU", c["AU"]),
+             ("EMRAT", c["EMRAT"])]
+    for k, v in GM_PLANETS.items():
+        names.append((k, v if k == "GMS" else v * model.planet_mass_scale))
+    for k in ("J2E", "J3E", "J4E", "J2SUN", "RE", "ASUN"):
+        names.append((k, c[k]))
+    # filler constants up to the first table's 400 names; the asteroid masses follow, in the
+    # second name table (real DE440 files also hold more than 400 constants)
+    k = 0
+    while len(names) < ASCII_FILLER_CONSTANTS:
+        names.append(("XC%04d" % k, float(k)))
+        k += 1
+    for a in model.asteroids:
+        names.append(("MA%04d" % a["num"], a["gm"]))
+    return names
+
+
+def write_ascii_ephemeris(path, seed=0, planet_mass_scale=1.0, constants=None):
+    """Write the synthetic planets as a JPL binary ephemeris (.440 layout) covering JD 2451545
+    +/- 10 000 d in 32-day blocks; returns the model.  The Sun, planets and EMB are
+    barycentric and the Moon is geocentric, positions in km, as in the JPL files."""
+    model = SyntheticModel(seed, planet_mass_scale)
+    c = dict(CONSTANTS, **(constants or {}))
+    offs, words = ascii_column_offsets()
+    rec_bytes = 8 * words
+    beg = JD_J2000 - COVERAGE_DAYS
+    nblk = int(round(2 * COVERAGE_DAYS / ASCII_BLOCK_DAYS))
+    end = beg + nblk * ASCII_BLOCK_DAYS
+    consts = _ascii_constants(model, constants)
+    num = len(consts)
+    head = bytearray(rec_bytes)
+    for i, line in enumerate(ASCII_TITLE):
+        head[84 * i:84 * (i + 1)] = line.ljust(84).encode("ascii")
+    names = [n.ljust(6).encode("ascii") for n, _ in consts]
+    for p in range(400):
+        head[0x00FC + 6 * p:0x00FC + 6 * (p + 1)] = names[p]
+    o = 0x0A5C
+    struct.pack_into("<dddidd", head, o, beg, end, ASCII_BLOCK_DAYS, num, c["AU"], c["EMRAT"])
+    o += 44
+    for k in range(12):
+        struct.pack_into("<iii", head, o, offs[k], ASCII_COLUMNS[k][1], ASCII_COLUMNS[k][2])
+        o += 12
+    struct.pack_into("<i", head, o, 440)
+    o += 4
+    struct.pack_into("<iii", head, o, offs[12], ASCII_COLUMNS[12][1], ASCII_COLUMNS[12][2])
+    o += 12
+    assert o == 0x0B28
+    for p in range(400, num):
+        head[o:o + 6] = names[p]
+        o += 6
+    for k in (13, 14):
+        struct.pack_into("<iii", head, o, offs[k], ASCII_COLUMNS[k][1], ASCII_COLUMNS[k][2])
+        o += 12
+    cons = np.zeros(words)
+    cons[:num] = [v for _, v in consts]
+
+    data = np.zeros((nblk, words))
+    data[:, 0] = beg + ASCII_BLOCK_DAYS * np.arange(nblk)
+    data[:, 1] = data[:, 0] + ASCII_BLOCK_DAYS
+    t0 = beg - JD_J2000
+    truth = {
+        "MER": lambda tt: model.planet(1, tt)[0], "VEN": lambda tt: model.planet(2, tt)[0],
+        "EMB": lambda tt: model.planet(3, tt)[0], "MAR": lambda tt: model.planet(4, tt)[0],
+        "JUP": lambda tt: model.planet(5, tt)[0], "SAT": lambda tt: model.planet(6, tt)[0],
+        "URA": lambda tt: model.planet(7, tt)[0], "NEP": lambda tt: model.planet(8, tt)[0],
+        "PLU": lambda tt: model.planet(9, tt)[0], "LUN": lambda tt: model.moon_geocentric(tt)[0],
+        "SUN": lambda tt: model.sun(tt)[0],
+    }
+    rng = np.random.default_rng(1000 + seed)
+    for k, (name, ncf, niv, ncm) in enumerate(ASCII_COLUMNS):
+        if ncf == 0:
+            continue
+        if name in truth:
+            fn = truth[name]
+            _, _, cf = chebyshev_fit_records(lambda tt: fn(tt) * AU_KM, t0, ASCII_BLOCK_DAYS / niv, nblk * niv, ncf)
+            blocks = cf.reshape(nblk, niv * ncm * ncf)  # (block, sub-interval, component, coefficient)
+        else:
+            # nutations / librations: not used by the force model; bounded filler values
+            blocks = rng.normal(0.0, 1e-3, (nblk, niv * ncm * ncf))
+        data[:, offs[k] - 1:offs[k] - 1 + niv * ncm * ncf] = blocks
+    with open(path, "wb") as f:
+        f.write(bytes(head))
+        f.write(cons.astype("<f8").tobytes())
+        f.write(data.astype("<f8").tobytes())
+    return model
+
+
 def default_paths():
     root = os.environ.get("ASSIST_AMD_DATA", os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "data"))
     return os.path.join(root, "synthetic_planets.bsp"), os.path.join(root, "synthetic_asteroids.bsp")
+
+
+def ensure_synthetic_ascii(seed=0):
+    """Path of the seed-``seed`` synthetic JPL binary planets file (the asteroids stay in the
+    SPK file of ``ensure_synthetic_ephemeris``), generating it when absent."""
+    pp, _ = default_paths()
+    path = pp.replace(".bsp", ".440" if seed == 0 else "_s%d.440" % seed)
+    if not os.path.exists(path):
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp%d" % os.getpid()
+        write_ascii_ephemeris(tmp, seed)
+        os.replace(tmp, path)
+    return path
 
 
 def ensure_synthetic_ephemeris(seed=0):

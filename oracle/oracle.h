@@ -7,6 +7,8 @@
  *
  *   spk.c    DAF/SPK loading and type-2 Chebyshev evaluation
  *            (reference src/spk.c:133-210, 267-547, 550-693, 696-789)
+ *   ascii.c  JPL binary (.440/.441) planets: reader and evaluation
+ *            (reference src/ascii_ephem.c:27-65, 105-252, 275-384)
  *   forces.c the force model, term by term, in the reference's order
  *            (reference src/forces.c:49-1983)
  *   ias15.c  IAS15 restated from Rein & Spiegel (2015), MNRAS 446, 1424
@@ -51,9 +53,12 @@ typedef struct {
     size_t nbytes;
 } orc_spk;
 
+typedef struct orc_ascii orc_ascii; /* JPL binary (.440/.441) planets file, ascii.c */
+
 typedef struct {
     double jd_ref;
     orc_spk *planets, *asteroids;
+    orc_ascii *ascii; /* planets from a JPL binary file (then planets == NULL) */
     int target_index[ORC_NPLANETS]; /* index into planets->targets, -1 if absent */
     int emb_index;
     double AU, EMRAT, J2E, J3E, J4E, J2SUN, RE, CLIGHT, ASUN;
@@ -99,6 +104,17 @@ int orc_ephem_num_asteroids(const orc_ephem *e);
 int orc_all_ephem(const orc_ephem *e, int i, double t, double out[10]);
 void orc_time_bounds(const orc_ephem *e, double *t_beg, double *t_end);
 int orc_spk_eval_target(const orc_ephem *e, int which_file, int target_idx, double t, double out[9]);
+
+/* ---- JPL binary planets (ascii.c) ---- */
+int orc_ascii_signature(const unsigned char *buf, size_t n);
+orc_ascii *orc_ascii_load(const char *path);
+void orc_ascii_free(orc_ascii *a);
+int orc_ascii_find_constant(const orc_ascii *a, const char *name, double *out);
+int orc_ascii_calc(const orc_ascii *a, double jd_ref, double jd_rel, int body, double *GM, double o[9]);
+double orc_ascii_emrat(const orc_ascii *a);
+void orc_ascii_bounds(const orc_ascii *a, double *beg, double *end);
+/* 0 J2E, 1 J3E, 2 J4E, 3 J2SUN, 4 AU, 5 RE, 6 CLIGHT, 7 ASUN */
+double orc_ascii_constant(const orc_ascii *a, int which);
 
 /* ---- forces ---- */
 void orc_opts_default(orc_opts *o);

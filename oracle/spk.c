@@ -182,10 +182,55 @@ static int find_target(const orc_spk *s, int code) {
 
 static const int naif_by_assist[ORC_NPLANETS] = {10, 1, 2, 399, 301, 4, 5, 6, 7, 8, 9};
 
+/* Planets from a JPL binary file: constants from its table, asteroid masses from its MAnnnn
+ * constants (reference src/assist.c:291-327). */
+static orc_ephem *ascii_ephem_load(orc_ephem *e, const char *planets_path, const char *asteroids_path, int *err) {
+    e->ascii = orc_ascii_load(planets_path);
+    if (!e->ascii) { *err = ORC_ERROR_EPHEM_FILE; free(e); return NULL; }
+    if (asteroids_path) e->asteroids = spk_load(asteroids_path);
+    for (int k = 0; k < ORC_NPLANETS; k++) e->target_index[k] = -1;
+    e->emb_index = -1;
+    e->J2E = orc_ascii_constant(e->ascii, 0);
+    e->J3E = orc_ascii_constant(e->ascii, 1);
+    e->J4E = orc_ascii_constant(e->ascii, 2);
+    e->J2SUN = orc_ascii_constant(e->ascii, 3);
+    e->AU = orc_ascii_constant(e->ascii, 4);
+    e->RE = orc_ascii_constant(e->ascii, 5);
+    e->CLIGHT = orc_ascii_constant(e->ascii, 6);
+    e->ASUN = orc_ascii_constant(e->ascii, 7);
+    e->EMRAT = orc_ascii_emrat(e->ascii);
+    e->Re_eq = e->RE / e->AU;
+    e->Rs_eq = e->ASUN / e->AU;
+    e->c_AU_per_day = (e->CLIGHT / e->AU) * 86400.0;
+    e->c_squared = e->c_AU_per_day * e->c_AU_per_day;
+    e->over_c_squared = 1.0 / e->c_squared;
+    if (e->asteroids) {
+        for (int m = 0; m < e->asteroids->num; m++) {
+            orc_target *t = &e->asteroids->targets[m];
+            if (t->code >= 2000000) {
+                char key[16]; /* only the first 6 characters take part in the match */
+                snprintf(key, sizeof key, "MA%04d", t->code - 2000000);
+                double gm = 0.0;
+                if (orc_ascii_find_constant(e->ascii, key, &gm)) t->mass = gm;
+            }
+        }
+    }
+    return e;
+}
+
 orc_ephem *orc_ephem_load(const char *planets_path, const char *asteroids_path, int *err) {
     orc_ephem *e = calloc(1, sizeof(orc_ephem));
     e->jd_ref = 2451545.0;
     *err = ORC_SUCCESS;
+    {
+        /* format detection, reference src/assist.c:108-129 */
+        unsigned char head[0x00FC + 18];
+        FILE *f = fopen(planets_path, "rb");
+        size_t got = f ? fread(head, 1, sizeof head, f) : 0;
+        if (f) fclose(f);
+        if (!(got >= 8 && strncmp((const char *)head, "DAF/SPK ", 8) == 0) && orc_ascii_signature(head, got))
+            return ascii_ephem_load(e, planets_path, asteroids_path, err);
+    }
     e->planets = spk_load(planets_path);
     if (!e->planets) { *err = ORC_ERROR_EPHEM_FILE; free(e); return NULL; }
     if (asteroids_path) e->asteroids = spk_load(asteroids_path);
@@ -210,6 +255,7 @@ void orc_ephem_free(orc_ephem *e) {
     if (!e) return;
     spk_free(e->planets);
     spk_free(e->asteroids);
+    orc_ascii_free(e->ascii);
     free(e);
 }
 
@@ -218,6 +264,7 @@ int orc_ephem_num_asteroids(const orc_ephem *e) { return e->asteroids ? e->aster
 /* reference src/assist.c:524-554 */
 void orc_time_bounds(const orc_ephem *e, double *t_beg, double *t_end) {
     double beg = -INFINITY, end = INFINITY;
+    if (e->ascii) orc_ascii_bounds(e->ascii, &beg, &end);
     const orc_spk *sets[2] = {e->planets, e->asteroids};
     for (int f = 0; f < 2; f++) {
         if (!sets[f]) continue;
@@ -300,6 +347,7 @@ static int asteroid_pos(const orc_spk *s, double jd_ref, double jd_rel, int m, d
 /* reference src/spk.c:550-610 and :632-693 */
 static int planet_calc(const orc_ephem *e, double jd_ref, double t, int body, double *GM, double o[9]) {
     if (body < 0 || body >= ORC_NPLANETS) return ORC_ERROR_NEPHEM;
+    if (e->ascii) return orc_ascii_calc(e->ascii, jd_ref, t, body, GM, o);
     const orc_spk *pl = e->planets;
     int code = naif_by_assist[body];
     int idx = e->target_index[body];

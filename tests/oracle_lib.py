@@ -28,7 +28,7 @@ class OrcIas15Cfg(C.Structure):
 
 
 class OrcEphem(C.Structure):
-    _fields_ = [("jd_ref", C.c_double), ("planets", C.c_void_p), ("asteroids", C.c_void_p),
+    _fields_ = [("jd_ref", C.c_double), ("planets", C.c_void_p), ("asteroids", C.c_void_p), ("ascii", C.c_void_p),
                 ("target_index", C.c_int * 11), ("emb_index", C.c_int)] + [
         (k, C.c_double) for k in ("AU", "EMRAT", "J2E", "J3E", "J4E", "J2SUN", "RE", "CLIGHT", "ASUN",
                                   "Re_eq", "Rs_eq", "c_AU_per_day", "c_squared", "over_c_squared")]

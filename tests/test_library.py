@@ -23,8 +23,9 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
 def _declared_functions():
-    """Names of the functions include/assist.h declares (function-pointer members excluded)."""
-    txt = open(os.path.join(ROOT, "include", "assist.h")).read()
+    """Names of the functions include/*.h declare (function-pointer members excluded)."""
+    inc = os.path.join(ROOT, "include")
+    txt = "\n".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc)) if f.endswith(".h"))
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     txt = re.sub(r"//[^\n]*", "", txt)
     txt = re.sub(r"^\s*#[^\n]*", "", txt, flags=re.M)
