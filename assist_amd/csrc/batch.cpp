@@ -473,6 +473,7 @@ int tuned_shape(const IntegrateArgs& base, bool mixed, hipStream_t s, LaunchShap
 extern "C" {
 
 void assist_batch_config_default(struct assist_batch_config* cfg) {
+    memset(cfg, 0, sizeof *cfg);  // every field, including those added later, starts defined
     cfg->epsilon = 1e-9;
     cfg->min_dt = 0.0;
     cfg->dt_initial = 0.001;

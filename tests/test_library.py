@@ -186,6 +186,18 @@ def test_struct_mirrors_match_the_header():
             assert getattr(cls, f).offset == offs[c], (name, f)
 
 
+def test_batch_config_default_sets_every_field():
+    """assist_batch_config_default defines the whole struct, whatever the memory held before (C
+    callers declare it on the stack)."""
+    import assist_amd
+    from assist_amd.batch import BatchConfig
+    c = BatchConfig()
+    C.memset(C.addressof(c), 0x5A, C.sizeof(c))
+    assist_amd.clibassist.assist_batch_config_default(C.byref(c))
+    assert (c.epsilon, c.min_dt, c.dt_initial, c.max_steps, c.precision) == (1e-9, 0.0, 0.001, 1000000, 0)
+    assert (c.lanes_per_system, c.block_size, c.blocks_per_cu, c.waves_per_simd, c.reserved_) == (0, 0, 0, 0, 0)
+
+
 def test_gpu_entry_points_fail_without_gpu(ephem):
     from assist_amd import batch
     if batch.gpu_count() > 0:
