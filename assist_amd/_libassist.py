@@ -18,7 +18,7 @@ if _override:
 else:
     _path = os.path.join(pymodulespath, "libassist.so")
 if not os.path.exists(_path):
-    raise ImportError("libassist.so not found at %s; build it with `python -m assist_amd.build`" % _path)
+    raise ImportError("libassist.so not found at %s; build it with `python assist_amd/build.py`" % _path)
 clibassist = cdll.LoadLibrary(_path)
 __libpath__ = _path
 

@@ -1,7 +1,9 @@
 """Build libassist.so in-tree: HIP kernels for gfx950 plus the C/C++ host library.
 
-    python -m assist_amd.build            # build (always recompiles)
-    python -m assist_amd.build --check    # build only if sources are newer
+    python assist_amd/build.py            # build (always recompiles)
+    python assist_amd/build.py --check    # build only if sources are newer
+
+(It imports nothing from the package, whose import needs the library this builds.)
 
 The kernels are compiled with hipcc for gfx950 only, with packed FP32
 instructions disabled (-Xclang -target-feature -Xclang -packed-fp32-ops); the
