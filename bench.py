@@ -143,9 +143,9 @@ def copy_calibration(mib):
     dst.free()
 
 
-def measured_traffic(config, precision, n):
+def measured_traffic(config, form, n):
     """HBM bytes per launch of the integration kernel from the committed counter profile
-    (profiles/ias15_kernel_traffic.md), or None when this workload was not profiled."""
+    (profiles/ias15_kernel_traffic.md), or None when this workload and form were not profiled."""
     path = os.path.join(ROOT, "profiles", "ias15_kernel_traffic.md")
     try:
         txt = open(path).read()
@@ -154,7 +154,7 @@ def measured_traffic(config, precision, n):
     for line in txt.splitlines():
         if line.startswith("traffic_json:"):
             for row in json.loads(line.split(":", 1)[1]):
-                if row["config"] == config and row["precision"] == precision and row["systems"] == n:
+                if row["config"] == config and row["form"] == form and row["systems"] == n:
                     return row["bytes_per_launch"]
     return None
 
