@@ -244,15 +244,20 @@ int get_device_ephem(const assist_ephem* e, const DevEphem** out, DevEphem* host
 
 int make_force_opts(const assist_extras* x, ForceOpts* fo) {
     if (x) {
-        if (x->geocentric) return ASSIST_BATCH_ERROR_UNSUPPORTED;
-        if (x->gr_eih_sources > 1 && (x->forces & ASSIST_FORCE_GR_EIH)) return ASSIST_BATCH_ERROR_UNSUPPORTED;
+        // EIH sources beyond the 11 planet-file bodies would be asteroids, whose velocities the
+        // reference does not have (NaN, reference src/forces.c:222-223)
+        if ((x->forces & ASSIST_FORCE_GR_EIH) && (x->gr_eih_sources < 0 || x->gr_eih_sources > ASSIST_BODY_NPLANETS))
+            return ASSIST_BATCH_ERROR_UNSUPPORTED;
+        if (x->geocentric != 0 && x->geocentric != 1) return ASSIST_BATCH_ERROR_ARG;
         fo->forces = x->forces;
         fo->gr_eih_sources = x->gr_eih_sources;
         fo->alpha = x->alpha; fo->nk = x->nk; fo->nm = x->nm; fo->nn = x->nn; fo->r0 = x->r0;
+        fo->geocentric = x->geocentric;
     } else {
         fo->forces = 0x7F;
         fo->gr_eih_sources = 1;
         fo->alpha = 1.0; fo->nk = 0.0; fo->nm = 2.0; fo->nn = 5.093; fo->r0 = 1.0;
+        fo->geocentric = 0;
     }
     return ASSIST_SUCCESS;
 }
@@ -373,7 +378,7 @@ int run_ias15(IntegrateArgs a, bool mixed, const LaunchShape& sh, int blocks_per
     int dev = 0, ncu = 0, per_cu = 0;
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice") ||
         !hip_ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute") ||
-        !hip_ok(ias15_occupancy(sh.G, mixed, sh.waves, a.nvec, sh.threads, &per_cu), "occupancy"))
+        !hip_ok(ias15_occupancy(sh.G, mixed, sh.waves, ext_model(a.fo), a.nvec, sh.threads, &per_cu), "occupancy"))
         return ASSIST_BATCH_ERROR_DEVICE;
     if (per_cu < 1) per_cu = 1;
     if (blocks_per_cu_cap > 0 && blocks_per_cu_cap < per_cu) per_cu = blocks_per_cu_cap;
@@ -531,6 +536,7 @@ int assist_batch_forces(const struct assist_ephem* ephem, const struct assist_ex
     memset(&a, 0, sizeof a);
     int rc = make_force_opts(opts, &a.fo);
     if (rc) return rc;
+    if (ext_model(a.fo) && precision == ASSIST_PRECISION_MIXED) return ASSIST_BATCH_ERROR_UNSUPPORTED;
     rc = get_device_ephem(ephem, &a.E, nullptr);
     if (rc) return rc;
     const int nv = 1 + n_var;
@@ -627,8 +633,11 @@ int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const str
     if (shape.threads % 64 != 0 || shape.threads > kIas15Block || shape.threads % shape.G != 0) return ASSIST_BATCH_ERROR_ARG;
     shape.waves = cfg.waves_per_simd > 0 ? cfg.waves_per_simd : 1;
     if (shape.waves != 1 && shape.waves != 2) return ASSIST_BATCH_ERROR_ARG;
+    // the extended force model (geocentric, several EIH sources) runs in FP64 with the 1-wave kernel
+    const bool ext = ext_model(a.fo);
+    if (ext && (mixed || shape.waves != 1)) return ASSIST_BATCH_ERROR_UNSUPPORTED;
     const bool tune = cfg.lanes_per_system == 0 && cfg.block_size == 0 && cfg.waves_per_simd == 0 &&
-                      n >= kTuneMinSystems && !dense;
+                      n >= kTuneMinSystems && !dense && !ext;
     if (tune) {
         rc = tuned_shape(a, mixed, s, &shape);
         if (rc) return rc;

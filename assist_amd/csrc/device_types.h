@@ -69,8 +69,10 @@ struct DevEphem {
 
 struct ForceOpts {
     int forces;          // ASSIST_FORCE_* bitmask
-    int gr_eih_sources;  // 0 or 1 on the device
+    int gr_eih_sources;  // EIH sources: the first gr_eih_sources planet-file bodies (0..11)
     double alpha, nk, nm, nn, r0;  // Marsden g(r) shape
+    int geocentric;      // 1: particle states relative to the Earth (reference src/forces.c:72-87)
+    int pad_;
 };
 
 // IAS15 configuration for the batched integrator.

@@ -130,6 +130,15 @@ AHD int gbcast_i(int v) {
 #endif
     return v;
 }
+// Value of lane `src` of each group, with a run-time source lane (ds_bpermute).
+template <int G>
+AHD double gshfl(double v, int src) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (G > 1) return __shfl(v, src, G);
+#endif
+    (void)src;
+    return v;
+}
 template <int G>
 AHD int gor(int v) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -177,13 +186,14 @@ template <typename D>
 struct ChebOut {
     D x, y, z;     // km
     D vx, vy, vz;  // km/day (only when requested)
+    D ax, ay, az;  // km/day^2 (only when requested)
 };
 
 struct alignas(16) Pair { double x, y; };
 
 // The device evaluates all kCoefPad terms of the zero-padded record with every load issued
 // up front; the operation-count instantiation (D != double) counts the file's P terms.
-template <typename D, bool Vel>
+template <typename D, bool Vel, bool Acc = false>
 AHD ChebOut<D> cheb_eval(const DevSlot& S, D t) {
     const D fi = (D(S.c_init) + t) * D(S.inv_intlen);
     int idx = (int)value_of(fi);
@@ -215,6 +225,9 @@ AHD ChebOut<D> cheb_eval(const DevSlot& S, D t) {
     ChebOut<D> o;
     D S0 = D(0.0), S1 = D(1.0);
     D vx = D(cx[1]), vy = D(cy[1]), vz = D(cz[1]);
+    // second derivatives U_p = d^2 T_p / dz^2: U_0 = U_1 = 0 (reference src/spk.c:520-530)
+    D U0 = D(0.0), U1 = D(0.0);
+    D ax = D(0.0), ay = D(0.0), az = D(0.0);
 #pragma unroll
     for (int p = 2; p < kCoefPad; p++) {
         if (p < P) {
@@ -222,11 +235,19 @@ AHD ChebOut<D> cheb_eval(const DevSlot& S, D t) {
             x = fma(D(cx[p]), T2, x);
             y = fma(D(cy[p]), T2, y);
             w = fma(D(cz[p]), T2, w);
-            if (Vel) {
+            if (Vel || Acc) {
                 const D S2 = fma(z2, S1, fma(D(2.0), T1, -S0));
                 vx = fma(D(cx[p]), S2, vx);
                 vy = fma(D(cy[p]), S2, vy);
                 vz = fma(D(cz[p]), S2, vz);
+                if (Acc) {
+                    const D U2 = fma(z2, U1, fma(D(4.0), S1, -U0));
+                    ax = fma(D(cx[p]), U2, ax);
+                    ay = fma(D(cy[p]), U2, ay);
+                    az = fma(D(cz[p]), U2, az);
+                    U0 = U1;
+                    U1 = U2;
+                }
                 S0 = S1;
                 S1 = S2;
             }
@@ -234,12 +255,20 @@ AHD ChebOut<D> cheb_eval(const DevSlot& S, D t) {
             T1 = T2;
         }
     }
-    if (Vel) {
+    if (Vel || Acc) {
         o.vx = vx * D(c[1]);
         o.vy = vy * D(c[1]);
         o.vz = vz * D(c[1]);
     } else {
         o.vx = o.vy = o.vz = D(0.0);
+    }
+    if (Acc) {
+        const D c2 = D(c[1]) * D(c[1]);
+        o.ax = ax * c2;
+        o.ay = ay * c2;
+        o.az = az * c2;
+    } else {
+        o.ax = o.ay = o.az = D(0.0);
     }
     o.x = x;
     o.y = y;
@@ -534,6 +563,81 @@ AHD void term_eih_sun(const DevEphem& E, const D xr[3], const D vr[3], const D x
     }
 }
 
+// One EIH source j of reference src/forces.c:1319-1499 (real particle) and :1508-1982
+// (partials), beta = gamma = 1, in FP64, for the extended force model (several sources and/or
+// the geocentric frame):
+//   xs, vs   source position and velocity relative to the frame origin (xj - xo, vj - vo);
+//   S0, dS0  sum_k GM_k / r_ik over the 11 planet-file bodies and its gradient (all sources);
+//   S1, aj   sum_{k != j} GM_k / r_jk and the source's Newtonian acceleration from the same
+//            bodies, from barycentric positions;
+//   ao       the frame origin's acceleration (zero for barycentric): term 6 uses aj - ao, term 8
+//            uses aj, as the reference does.
+// Term 7 and 8 contributions go to t78 (the reference adds their sums over the sources after the
+// source loop); the rest goes to acc.
+AHD void term_eih_source(const DevEphem& E, const double xr[3], const double vr[3], const double xs[3],
+                         const double vs[3], double GMj, double S0, const double dS0[3], double S1,
+                         const double aj[3], const double ao[3], bool need_jac, double acc[3], double t78[3],
+                         Jac<double>& J) {
+    const double oc2 = E.over_c_squared;
+    const double dij[3] = {xr[0] - xs[0], xr[1] - xs[1], xr[2] - xs[2]};
+    const double rij2 = dij[0] * dij[0] + dij[1] * dij[1] + dij[2] * dij[2];
+    const double ir = inv_sqrt(rij2);
+    const double ir2 = ir * ir;
+    const double ir3 = ir2 * ir;
+    const double prefacij = GMj * ir3;
+    const double vi2 = vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2];
+    const double term2 = oc2 * vi2;
+    const double vj2 = vs[0] * vs[0] + vs[1] * vs[1] + vs[2] * vs[2];
+    const double term3 = 2.0 * oc2 * vj2;
+    const double vidotvj = vr[0] * vs[0] + vr[1] * vs[1] + vr[2] * vs[2];
+    const double term4 = -4.0 * oc2 * vidotvj;
+    const double rijdotvj = dij[0] * vs[0] + dij[1] * vs[1] + dij[2] * vs[2];
+    const double term5 = -1.5 * oc2 * (rijdotvj * rijdotvj) * ir2;
+    double fv[3], rel[3], am[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        fv[c] = 4.0 * vr[c] - 3.0 * vs[c];
+        rel[c] = vr[c] - vs[c];
+        am[c] = aj[c] - ao[c];
+    }
+    const double f = dij[0] * fv[0] + dij[1] * fv[1] + dij[2] * fv[2];
+    const double pf = prefacij * f;
+    const double term0 = S0 * (-4.0 * oc2);
+    const double term1 = S1 * (-oc2);
+    const double term6 = -0.5 * oc2 * (dij[0] * am[0] + dij[1] * am[1] + dij[2] * am[2]);
+    const double t8 = GMj * ir * 3.5;
+    const double factor = term0 + term1 + term2 + term3 + term4 + term5 + term6;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        acc[c] += -prefacij * dij[c] * factor;
+        t78[c] += pf * rel[c] + t8 * aj[c];
+    }
+    if (!need_jac) return;
+    const double g5 = -3.0 * GMj * (ir3 * ir2);
+    const double dpref[3] = {g5 * dij[0], g5 * dij[1], g5 * dij[2]};
+    const double t5f = 3.0 * oc2 * rijdotvj * ir;
+    double dfx[3], dfv[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        dfx[q] = dS0[q] * (-4.0 * oc2) - t5f * (vs[q] * ir - rijdotvj * dij[q] * ir3) - 0.5 * oc2 * am[q];
+        dfv[q] = 2.0 * oc2 * vr[q] - 4.0 * oc2 * vs[q];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const double d7x = dpref[q] * f * rel[c] + prefacij * fv[q] * rel[c];
+            double d7v = prefacij * (4.0 * dij[q]) * rel[c];
+            if (q == c) d7v += pf;
+            const double d8 = -GMj * aj[c] * ir3 * dij[q] * 3.5;
+            double v = -dpref[q] * dij[c] * factor - prefacij * dij[c] * dfx[q];
+            if (q == c) v -= prefacij * factor;
+            J.x[c][q] += v + (d7x * oc2 + d8 * oc2);
+            J.v[c][q] += -prefacij * dij[c] * dfv[q] + d7v * oc2;
+        }
+    }
+}
+
 // GR potential form (reference src/forces.c:1059-1161), Sun only, in TS.
 template <typename D, typename TS>
 AHD void term_gr_potential(const DevEphem& E, const D xr[3], const D xs[3], D GMs, bool need_jac, D acc[3], Jac<TS>& J) {
@@ -599,12 +703,72 @@ struct TermScalar { using type = typename std::conditional<Mixed, typename Singl
 // Evaluates the force model for the real particle at (t, xr, vr) cooperatively over the G lanes
 // of a group; every lane receives the real acceleration and the Jacobian J.  `slots` and `gm` are
 // the ephemeris slot descriptors and masses (E.slot and E.GM, or copies of them in LDS).
-template <int G, bool Mixed, typename D>
+// The EIH sources of the extended model: sources j < gr_eih_sources are the planet-file bodies
+// in ASSIST order (Sun, Mercury, Venus, Earth, Moon, Mars, ..., Pluto).  For each source the
+// lanes that own the other bodies add their GM/r_jk and Newtonian pulls (barycentric
+// positions, as reference src/forces.c:1408-1432), the group sums them, and every lane then
+// evaluates the source's terms with term_eih_source.  px/pv hold this lane's barycentric slot
+// positions and velocities in au, au/d.
+template <int G, int NR>
+AHD void eih_sources(const DevEphem& E, const ForceOpts& O, const double* gm, int lane, const double xr[3],
+                     const double vr[3], const double (&px)[NR][3], const double (&pv)[NR][3], const double xo[3],
+                     const double vo[3], const double ao[3], double S0, const double dS0[3], bool need_jac,
+                     double acc[3], Jac<double>& J) {
+    const int nsrc = O.gr_eih_sources < kNumPlanets ? O.gr_eih_sources : kNumPlanets;
+    double t78[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < kNumPlanets; j++) {
+        if (j >= nsrc) break;
+        const int sj = j < 3 ? j : j + 1;  // body -> slot (the EMB's slot 3 is skipped)
+        double xj[3], vj[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            xj[c] = gshfl<G>(px[sj / G][c], sj % G);
+            vj[c] = gshfl<G>(pv[sj / G][c], sj % G);
+        }
+        double s1 = 0.0, aj[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int s = r * G + lane;
+            if (s >= kFirstAsteroidSlot || s == kSlotEMB) continue;
+            const int b = slot_body(s);
+            if (b == j) continue;
+            const double dx = xj[0] - px[r][0], dy = xj[1] - px[r][1], dz = xj[2] - px[r][2];
+            const double q2 = dx * dx + dy * dy + dz * dz;
+            const double iq = inv_sqrt(q2);
+            s1 += gm[b] * iq;
+            const double f = gm[b] * (iq * iq * iq);
+            aj[0] -= f * dx; aj[1] -= f * dy; aj[2] -= f * dz;
+        }
+        if constexpr (G > 1) {
+            s1 = gsum<G>(s1);
+#pragma unroll
+            for (int c = 0; c < 3; c++) aj[c] = gsum<G>(aj[c]);
+        }
+        const double xs[3] = {xj[0] - xo[0], xj[1] - xo[1], xj[2] - xo[2]};
+        const double vs[3] = {vj[0] - vo[0], vj[1] - vo[1], vj[2] - vo[2]};
+        term_eih_source(E, xr, vr, xs, vs, gm[j], S0, dS0, s1, aj, ao, need_jac, acc, t78, J);
+    }
+    const double oc2 = E.over_c_squared;
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[c] += t78[c] * oc2;
+}
+
+//
+// Ext selects the extended model (FP64 only): several EIH sources (gr_eih_sources up to 11) and
+// the geocentric frame.  Every planet-file slot is then evaluated with velocity and acceleration;
+// body positions are shifted by the Earth's (xk - xo), so every particle-body difference is the
+// reference's x + (xo - xk) bit for bit; each source's planet sums come from barycentric
+// positions, split over the lanes that own the bodies; and the Earth's acceleration is removed
+// from the real particle's at the end (the indirect term, reference src/forces.c:157-172).
+template <int G, bool Mixed, typename D, bool Ext = false>
 AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, const ForceOpts& O, int lane, D t, const D xr[3],
                     const D vr[3], const double A[3], bool ng, bool need_jac, EvalResult<D>& out,
                     Jac<typename TermScalar<D, Mixed>::type>& J) {
     using TS = typename TermScalar<D, Mixed>::type;
+    static_assert(!Ext || (!Mixed && std::is_same<D, double>::value), "the extended model is FP64 only");
     constexpr int NR = (kNumSlots + G - 1) / G;
+    constexpr int NRV = Ext ? NR : 1;  // rounds with velocities and accelerations kept
     const int forces = O.forces;
     const double tv = value_of(t);
     out.status = (tv >= E.t_beg && tv <= E.t_end) ? kOk : kCoverage;
@@ -614,21 +778,33 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
     // 1. Chebyshev evaluations of the slots owned by this lane; round 0 (which holds the Sun,
     //    slot 0, on lane 0) also evaluates velocities when the Sun's velocity is needed
     D px[NR][3];
+    D pv[NRV][3], pa[NRV][3];
     D sunv_l[3] = {D(0.0), D(0.0), D(0.0)};
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         const int s = r * G + lane;
         if (s < kNumSlots) {
             ChebOut<D> c;
-            if (r == 0 && want_sun_vel) {
+            if (Ext && r * G < kFirstAsteroidSlot) {
+                c = cheb_eval<D, true, Ext>(slots[s], t);
+                if (r == 0) { sunv_l[0] = c.vx; sunv_l[1] = c.vy; sunv_l[2] = c.vz; }
+            } else if (r == 0 && want_sun_vel) {
                 c = cheb_eval<D, true>(slots[s], t);
                 sunv_l[0] = c.vx; sunv_l[1] = c.vy; sunv_l[2] = c.vz;
             } else {
                 c = cheb_eval<D, false>(slots[s], t);
             }
             px[r][0] = c.x; px[r][1] = c.y; px[r][2] = c.z;
+            if constexpr (Ext) {
+                pv[r][0] = c.vx; pv[r][1] = c.vy; pv[r][2] = c.vz;
+                pa[r][0] = c.ax; pa[r][1] = c.ay; pa[r][2] = c.az;
+            }
         } else {
             px[r][0] = px[r][1] = px[r][2] = D(0.0);
+            if constexpr (Ext) {
+                pv[r][0] = pv[r][1] = pv[r][2] = D(0.0);
+                pa[r][0] = pa[r][1] = pa[r][2] = D(0.0);
+            }
         }
     }
     // 2. exchange Sun (au) and EMB (km); compose Earth, Moon and asteroids
@@ -640,6 +816,14 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
         sunv[c] = gbcast<G, 0>(sunv_l[c] * inv_au);
         emb[c] = gbcast<G, kSlotEMB % G>(px[kSlotEMB / G][c]);
     }
+    D embv[3], emba[3];
+    if constexpr (Ext) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            embv[c] = gbcast<G, kSlotEMB % G>(pv[kSlotEMB / G][c]);
+            emba[c] = gbcast<G, kSlotEMB % G>(pa[kSlotEMB / G][c]);
+        }
+    }
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         const int s = r * G + lane;
@@ -648,6 +832,13 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
             const bool em = (s == kSlotEarth || s == kSlotMoon);
 #pragma unroll
             for (int c = 0; c < 3; c++) px[r][c] = (em ? px[r][c] + emb[c] : px[r][c]) * inv_au;
+            if constexpr (Ext) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    pv[r][c] = (em ? pv[r][c] + embv[c] : pv[r][c]) * inv_au;
+                    pa[r][c] = (em ? pa[r][c] + emba[c] : pa[r][c]) * inv_au;
+                }
+            }
         } else {
 #pragma unroll
             for (int c = 0; c < 3; c++) px[r][c] = fma(px[r][c], D(E.inv_au_ast), sun[c]);
@@ -656,6 +847,24 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
     D earth[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) earth[c] = gbcast<G, kSlotEarth % G>(px[kSlotEarth / G][c]);
+    // frame origin: the Earth's barycentric state when geocentric, else zero
+    D xo[3] = {D(0.0), D(0.0), D(0.0)}, vo[3] = {D(0.0), D(0.0), D(0.0)}, ao[3] = {D(0.0), D(0.0), D(0.0)};
+    if constexpr (Ext) {
+        if (O.geocentric) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                xo[c] = earth[c];
+                vo[c] = gbcast<G, kSlotEarth % G>(pv[kSlotEarth / G][c]);
+                ao[c] = gbcast<G, kSlotEarth % G>(pa[kSlotEarth / G][c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            sun[c] = sun[c] - xo[c];
+            sunv[c] = sunv[c] - vo[c];
+            earth[c] = earth[c] - xo[c];
+        }
+    }
 
     // 3. per-body point masses, Jacobian and EIH planet sums on the owning lane
     D ad[3] = {D(0.0), D(0.0), D(0.0)};  // direct acceleration, FP64 bodies
@@ -671,7 +880,10 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
         if (b >= nbody) continue;
         const D GM = D(gm[b]);
         const bool allowed = (b == 0) ? (forces & F_SUN) : (b < kNumPlanets ? (forces & F_PLANETS) : (forces & F_ASTEROIDS));
-        const D dx = xr[0] - px[r][0], dy = xr[1] - px[r][1], dz = xr[2] - px[r][2];
+        // (with the extended model, xo is the frame origin: x - (xk - xo) == x + (xo - xk))
+        const D dx = Ext ? xr[0] - (px[r][0] - xo[0]) : xr[0] - px[r][0];
+        const D dy = Ext ? xr[1] - (px[r][1] - xo[1]) : xr[1] - px[r][1];
+        const D dz = Ext ? xr[2] - (px[r][2] - xo[2]) : xr[2] - px[r][2];
         // Mixed form: an asteroid's point mass runs in FP32 unless the particle is within
         // kFp32MinDist of it, where the FP32 round-off of the term would reach IAS15's
         // error estimate (see kFp32MinDist).
@@ -708,7 +920,7 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
         if (eih && b < kNumPlanets) {
             S0 += GM * ir;
             dS0[0] += ax; dS0[1] += ay; dS0[2] += az;
-            if (b != 0) {
+            if (!Ext && b != 0) {
                 const D sx = sun[0] - px[r][0], sy = sun[1] - px[r][1], sz = sun[2] - px[r][2];
                 const D q2 = sx * sx + sy * sy + sz * sz;
                 const D iq = inv_sqrt(q2);
@@ -739,7 +951,11 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
     if ((forces & F_NONGRAV) && ng) term_nongrav<D, TS>(O, xr, vr, sun, sunv, A, need_jac, acc, J);
     if (forces & F_EARTH_H) term_earth_J2J4<D, TS>(E, xr, earth, D(E.GM[3]), need_jac, acc, J);
     if (forces & F_SUN_H) term_solar_J2<D, TS>(E, xr, sun, GMs, need_jac, acc, J);
-    if (eih) term_eih_sun<D, TS>(E, xr, vr, sun, sunv, GMs, S0, dS0, S1, aj, need_jac, acc, J);
+    if constexpr (Ext) {
+        if (eih) eih_sources<G>(E, O, gm, lane, xr, vr, px, pv, xo, vo, ao, S0, dS0, need_jac, acc, J);
+    } else {
+        if (eih) term_eih_sun<D, TS>(E, xr, vr, sun, sunv, GMs, S0, dS0, S1, aj, need_jac, acc, J);
+    }
     if (forces & F_GR_POTENTIAL) term_gr_potential<D, TS>(E, xr, sun, GMs, need_jac, acc, J);
     if (forces & F_GR_SIMPLE) term_gr_simple<D, TS>(E, xr, vr, sun, sunv, GMs, need_jac, acc, J);
     if (forces & (F_SUN | F_PLANETS | F_ASTEROIDS)) {
@@ -751,6 +967,11 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
             J.x[1][2] += jd[4]; J.x[2][1] += jd[4];
             J.x[0][2] += jd[5]; J.x[2][0] += jd[5];
         }
+    }
+    if constexpr (Ext) {
+        // indirect term of the geocentric frame (real particle only; ao is zero otherwise)
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[c] -= ao[c];
     }
     out.acc[0] = acc[0]; out.acc[1] = acc[1]; out.acc[2] = acc[2];
 }

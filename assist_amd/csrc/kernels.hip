@@ -35,7 +35,7 @@ __constant__ double cPV[9 * 7] = IAS15_PV_INIT;
 // ------------------------------------------------------------------------------------------
 // batched force evaluation
 // ------------------------------------------------------------------------------------------
-template <int G, bool Mixed>
+template <int G, bool Mixed, bool Ext = false>
 __global__ void __launch_bounds__(256) batch_forces_kernel(ForcesArgs a) {
     using TS = typename TermScalar<double, Mixed>::type;
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) batch_forces_kernel(ForcesArgs a) {
     }
     EvalResult<double> R;
     Jac<TS> J;
-    force_eval<G, Mixed, double>(*a.E, a.E->slot, a.E->GM, a.fo, lane, a.t[sys], xr, vr, A, ng, nv > 1, R, J);
+    force_eval<G, Mixed, double, Ext>(*a.E, a.E->slot, a.E->GM, a.fo, lane, a.t[sys], xr, vr, A, ng, nv > 1, R, J);
     // each lane handles vectors lane, lane+G, ... (G may be smaller than nvec for parity runs)
     for (int v = lane; v < nv; v += G) {
         double x[3], vv[3], out[3];
@@ -121,7 +121,7 @@ __device__ __forceinline__ void predict_e(const double br[7], double q, double e
     e[6] = q7 * br[6];
 }
 
-template <int G, bool Mixed, int W>
+template <int G, bool Mixed, int W, bool Ext = false>
 __global__ void __launch_bounds__(kIas15Block) __attribute__((amdgpu_waves_per_eu(W, W)))
 ias15_kernel(IntegrateArgs args) {
     // The arguments are read through the kernel-argument segment pointer, which is
@@ -339,7 +339,7 @@ ias15_kernel(IntegrateArgs args) {
         EvalResult<double> R;
         Jac<TS> J;
         const ForceOpts fo = ka->fo;
-        force_eval<G, Mixed, double>(E, s_slots, s_gm, fo, lane, te, xr, vr, A, ng, nv > 1, R, J);
+        force_eval<G, Mixed, double, Ext>(E, s_slots, s_gm, fo, lane, te, xr, vr, A, ng, nv > 1, R, J);
         double at[3];
         own_accel<double, TS>(lane, R, J, xe, ve, dA, ng, at);
         evals++;
@@ -657,17 +657,27 @@ hipError_t launch_start_post(int n, const float* Y, const double* t0, const doub
 // ------------------------------------------------------------------------------------------
 // host launchers
 // ------------------------------------------------------------------------------------------
-template <int G, bool M>
+template <int G, bool M, bool X = false>
 static hipError_t launch_forces_t(const ForcesArgs& a, hipStream_t s) {
     const int threads = 256;
     const long long total = (long long)a.nsys * G;
     const int blocks = (int)((total + threads - 1) / threads);
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((batch_forces_kernel<G, M>), dim3(blocks), dim3(threads), 0, s, a);
+    hipLaunchKernelGGL((batch_forces_kernel<G, M, X>), dim3(blocks), dim3(threads), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_forces(const ForcesArgs& a, int G, bool mixed, hipStream_t s) {
+    if (ext_model(a.fo)) {
+        if (mixed) return hipErrorInvalidValue;
+        switch (G) {
+        case 1: return launch_forces_t<1, false, true>(a, s);
+        case 2: return launch_forces_t<2, false, true>(a, s);
+        case 4: return launch_forces_t<4, false, true>(a, s);
+        case 8: return launch_forces_t<8, false, true>(a, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (G * 2 + (mixed ? 1 : 0)) {
     case 2: return launch_forces_t<1, false>(a, s);
     case 3: return launch_forces_t<1, true>(a, s);
@@ -681,12 +691,22 @@ hipError_t launch_forces(const ForcesArgs& a, int G, bool mixed, hipStream_t s) 
     }
 }
 
-template <int G, bool M, int W>
+template <int G, bool M, int W, bool X = false>
 static const void* ias15_fn() {
-    return (const void*)ias15_kernel<G, M, W>;
+    return (const void*)ias15_kernel<G, M, W, X>;
 }
 
-static const void* ias15_lookup(int G, bool mixed, int waves) {
+static const void* ias15_lookup(int G, bool mixed, int waves, bool ext) {
+    if (ext) {  // extended force model: FP64, 1-wave variant
+        if (mixed || waves != 1) return nullptr;
+        switch (G) {
+        case 1: return ias15_fn<1, false, 1, true>();
+        case 2: return ias15_fn<2, false, 1, true>();
+        case 4: return ias15_fn<4, false, 1, true>();
+        case 8: return ias15_fn<8, false, 1, true>();
+        default: return nullptr;
+        }
+    }
     const int key = (G * 2 + (mixed ? 1 : 0)) * 4 + waves;
     switch (key) {
 #define CASE(g, m, w) case (g * 2 + m) * 4 + w: return ias15_fn<g, (bool)m, w>();
@@ -703,7 +723,7 @@ size_t ias15_lds_bytes(int G, int nvec, int threads, int waves) {
 }
 
 hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int waves, int blocks, int threads, hipStream_t s) {
-    const void* f = ias15_lookup(G, mixed, waves);
+    const void* f = ias15_lookup(G, mixed, waves, ext_model(a.fo));
     if (!f) return hipErrorInvalidValue;
     const size_t lds = ias15_lds_bytes(G, a.nvec, threads, waves);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -714,8 +734,8 @@ hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int waves, in
     return hipGetLastError();
 }
 
-hipError_t ias15_occupancy(int G, bool mixed, int waves, int nvec, int threads, int* blocks_per_cu) {
-    const void* f = ias15_lookup(G, mixed, waves);
+hipError_t ias15_occupancy(int G, bool mixed, int waves, bool ext, int nvec, int threads, int* blocks_per_cu) {
+    const void* f = ias15_lookup(G, mixed, waves, ext);
     if (!f) return hipErrorInvalidValue;
     const size_t lds = ias15_lds_bytes(G, nvec, threads, waves);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

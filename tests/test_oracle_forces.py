@@ -260,3 +260,110 @@ def test_zero_marsden_parameters_skip_variations(oracle, model):
     t, x = _states(model, 4, 11)
     cols = _jacobian_cols(oracle, t, x, 0x08, np.zeros((4, 3)))
     assert (cols == 0).all()
+
+
+# ---- extended model: several EIH sources, geocentric frame (reference src/forces.c:72-87,
+# 157-172, 1288-1983) ----
+
+def _eih_textbook(bodies, ri, vi, nsrc, c2):
+    """EIH acceleration of a massless particle with sources j < nsrc (Newhall, Standish &
+    Williams 1983, beta = gamma = 1): every source's bracket uses the particle's potential from
+    all 11 bodies and the source's from the others; the source's Newtonian acceleration comes
+    from the other 10 bodies."""
+    pot_i = sum(b[0] / np.linalg.norm(ri - b[1:4]) for b in bodies)
+    a = np.zeros(3)
+    t78 = np.zeros(3)
+    for j in range(nsrc):
+        rj, vj, gmj = bodies[j][1:4], bodies[j][4:7], bodies[j][0]
+        aj = np.zeros(3)
+        pot_j = 0.0
+        for k in range(11):
+            if k == j:
+                continue
+            dk = rj - bodies[k][1:4]
+            aj -= bodies[k][0] * dk / np.linalg.norm(dk) ** 3
+            pot_j += bodies[k][0] / np.linalg.norm(dk)
+        rij = ri - rj
+        r = np.linalg.norm(rij)
+        bracket = (-4 * pot_i - pot_j + vi @ vi + 2 * vj @ vj - 4 * vi @ vj - 1.5 * (rij @ vj / r) ** 2
+                   + 0.5 * (-rij) @ aj) / c2
+        a += -gmj * rij / r**3 * bracket
+        t78 += gmj / r**3 * (rij @ (4 * vi - 3 * vj)) * (vi - vj) / c2 + 3.5 * gmj * aj / r / c2
+    return a + t78
+
+
+@pytest.mark.parametrize("nsrc", [2, 5, 11])
+def test_eih_several_sources_match_textbook(oracle, model, nsrc):
+    t, x = _states(model, 12, 20 + nsrc)
+    a = _acc(oracle, t, x, 0x40, gr_eih_sources=nsrc)
+    c2 = oracle.ephem.c_squared
+    for i in range(len(t)):
+        bodies = [_body(oracle, b, t[i]) for b in range(11)]
+        ref = _eih_textbook(bodies, x[i, :3], x[i, 3:], nsrc, c2)
+        assert np.linalg.norm(a[i] - ref) <= 1e-12 * np.linalg.norm(ref), (nsrc, i)
+    # the sources add: the Sun alone is the first term of the sum
+    a1 = _acc(oracle, t, x, 0x40, gr_eih_sources=1)
+    assert np.abs(a - a1).max() > 0
+
+
+def _near_earth(oracle, model, n, seed):
+    """Geocentric states 0.002-0.05 au from the Earth moving at heliocentric-orbit speeds."""
+    rng = np.random.default_rng(seed)
+    t = rng.uniform(-3000, 3000, n)
+    x = np.zeros((n, 6))
+    for i in range(n):
+        d = rng.normal(size=3)
+        x[i, :3] = d / np.linalg.norm(d) * rng.uniform(0.002, 0.05)
+        x[i, 3:] = rng.normal(size=3) * 0.003
+    return t, x
+
+
+def test_geocentric_frame_is_the_shifted_barycentric_one(oracle, model):
+    """For the position-dependent terms and the Marsden and simple-GR terms (which use velocities
+    relative to the Sun) the geocentric acceleration plus the Earth's equals the barycentric
+    acceleration at the shifted state (the reference's frame offset and indirect term)."""
+    t, x = _near_earth(oracle, model, 16, 31)
+    A = np.random.default_rng(32).normal(size=(16, 3)) * 1e-12
+    for forces in (0x07, 0x10, 0x20, 0x08, 0x80, 0x100, 0x3F):
+        ag = _acc(oracle, t, x, forces, A if forces & 0x08 else None, geocentric=1)
+        xb = x.copy()
+        ae = np.zeros((len(t), 3))
+        for i in range(len(t)):
+            e = _body(oracle, 3, t[i])
+            xb[i, :3] += e[1:4]
+            xb[i, 3:] += e[4:7]
+            ae[i] = e[7:10]
+        ab = _acc(oracle, t, xb, forces, A if forces & 0x08 else None)
+        # the indirect term (the Sun's pull on the Earth, ~3e-4 au/d^2) is subtracted and added
+        # back, so its rounding sets the scale for the small terms
+        err = np.linalg.norm(ag + ae - ab, axis=-1) / (np.linalg.norm(ab, axis=-1) + np.linalg.norm(ae, axis=-1))
+        assert err.max() <= 1e-13, (forces, err.max())
+
+
+@pytest.mark.parametrize("case", ["eih11", "geocentric", "geocentric_eih11"])
+def test_extended_partials_are_finite_differences(oracle, model, case):
+    opts = dict(gr_eih_sources=11) if "eih11" in case else {}
+    forces = 0x40 if case == "eih11" else 0x7F & ~0x08
+    if "geocentric" in case:
+        opts["geocentric"] = 1
+        t, x = _near_earth(oracle, model, 4, 33)
+    else:
+        t, x = _states(model, 4, 34)
+    n = len(t)
+    st = np.zeros((n, 7, 6))
+    st[:, 0] = x
+    for k in range(6):
+        st[:, 1 + k, k] = 1.0
+    acc, s = oracle.forces(t, st, None, oracle.opts(forces=forces, **opts))
+    assert (s == 0).all()
+    cols = acc[:, 1:]
+    for i in range(n):
+        for k in range(6):
+            h = (1e-6 if k < 3 else 1e-8) * (np.linalg.norm(x[i, :3]) if k < 3 else np.linalg.norm(x[i, 3:]))
+            xp, xm = x[i].copy(), x[i].copy()
+            xp[k] += h
+            xm[k] -= h
+            ap = _acc(oracle, np.array([t[i]] * 2), np.stack([xp, xm]), forces, **opts)
+            fd = (ap[0] - ap[1]) / (2 * h)
+            scale = np.abs(cols[i]).max()
+            assert np.abs(cols[i, k] - fd).max() <= 1e-6 * scale, (case, i, k, cols[i, k], fd)
