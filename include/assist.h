@@ -269,7 +269,9 @@ int assist_gpu_device_count(void);
  * geocentric and the Marsden shape (alpha, nk, nm, nn, r0); NULL selects the defaults of
  * assist_init.  states: n x (1+n_var) x 6 (x,y,z,vx,vy,vz; real particle first), params:
  * n x (1+n_var) x 3 (A1,A2,A3 for the real particle, their variations for the variational
- * ones) or NULL.  acc: n x (1+n_var) x 3.  status: per system (ASSIST_ERROR_COVERAGE etc.). */
+ * ones) or NULL.  acc: n x (1+n_var) x 3.  status: per system (ASSIST_ERROR_COVERAGE etc.).
+ * geocentric = 1 (states relative to the Earth) and gr_eih_sources from 2 to 11 run in FP64
+ * only: with ASSIST_PRECISION_MIXED they return ASSIST_BATCH_ERROR_UNSUPPORTED. */
 int assist_batch_forces(const struct assist_ephem* ephem, const struct assist_extras* opts, int n, int n_var,
                         const double* t, const double* states, const double* params, double* acc, int* status,
                         int precision);
