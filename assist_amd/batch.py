@@ -115,6 +115,20 @@ class IntegrateResult:
     stats: BatchStats
 
 
+def get_particles(ephem, body, t):
+    """Batched ephemeris queries on the GPU: (out (n, 10) = GM, x, y, z, vx, vy, vz, ax, ay, az,
+    status (n,)) for ASSIST bodies ``body[q]`` at times ``t[q]`` (broadcast against each other)."""
+    body, t = np.broadcast_arrays(np.asarray(body, dtype=np.int32), np.asarray(t, dtype=np.float64))
+    body = np.ascontiguousarray(body.ravel())
+    t = np.ascontiguousarray(t.ravel())
+    n = body.size
+    out = np.zeros((n, 10))
+    st = np.zeros(n, dtype=np.int32)
+    _check(clibassist.assist_batch_get_particles(C.byref(ephem), n, body.ctypes.data_as(_ip), _d(t), _d(out),
+                                                 st.ctypes.data_as(_ip)))
+    return out, st
+
+
 def integrate(ephem, t0, t1, states, params=None, extras=None, config: BatchConfig | None = None) -> IntegrateResult:
     """IAS15 for every system from t0[i] to t1[i] on the GPU (host arrays; copies included)."""
     states = np.array(states, dtype=np.float64, copy=True, order="C")

@@ -657,6 +657,40 @@ int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const str
     return ASSIST_SUCCESS;
 }
 
+int assist_batch_get_particles_device(const struct assist_ephem* ephem, int n, const int* d_body, const double* d_t,
+                                      double* d_out, int* d_status, void* stream) {
+    if (n < 0 || (n > 0 && (!d_body || !d_t || !d_out))) return ASSIST_BATCH_ERROR_ARG;
+    if (n == 0) return ASSIST_SUCCESS;
+    const DevEphem* E = nullptr;
+    int rc = get_device_ephem(ephem, &E, nullptr);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (!hip_ok(launch_ephem_query(E, n, d_body, d_t, d_out, d_status, s), "launch(ephem_query)") ||
+        !hip_ok(hipStreamSynchronize(s), "ephem_query"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    return ASSIST_SUCCESS;
+}
+
+int assist_batch_get_particles(const struct assist_ephem* ephem, int n, const int* body, const double* t, double* out,
+                               int* status) {
+    if (n < 0 || (n > 0 && (!body || !t || !out))) return ASSIST_BATCH_ERROR_ARG;
+    if (n == 0) return ASSIST_SUCCESS;
+    if (assist_gpu_device_count() <= 0) return ASSIST_BATCH_ERROR_DEVICE;
+    const size_t ns = (size_t)n;
+    DevBuf<int> db, dst;
+    DevBuf<double> dt, dout;
+    if (!db.alloc(ns) || !dt.alloc(ns) || !dout.alloc(ns * 10) || !dst.alloc(ns)) return ASSIST_BATCH_ERROR_DEVICE;
+    if (!hip_ok(hipMemcpy(db.p, body, ns * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy") ||
+        !hip_ok(hipMemcpy(dt.p, t, ns * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    const int rc = assist_batch_get_particles_device(ephem, n, db.p, dt.p, dout.p, dst.p, nullptr);
+    if (rc) return rc;
+    if (!hip_ok(hipMemcpy(out, dout.p, ns * 10 * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy") ||
+        (status && !hip_ok(hipMemcpy(status, dst.p, ns * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy")))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    return ASSIST_SUCCESS;
+}
+
 int assist_batch_integrate(const struct assist_ephem* ephem, const struct assist_extras* opts,
                            const struct assist_batch_config* cfg, int n, int n_var, const double* t0,
                            const double* t1, double* states, const double* params, int* status, long long* steps,

@@ -92,5 +92,7 @@ hipError_t ias15_occupancy(int G, bool mixed, int waves, bool ext, int nvec, int
 inline bool ext_model(const ForceOpts& f) { return f.geocentric != 0 || ((f.forces & 0x40) && f.gr_eih_sources > 1); }
 size_t ias15_lds_bytes(int G, int nvec, int threads, int waves);
 hipError_t launch_set_counter(int* q, int v, hipStream_t s);
+hipError_t launch_ephem_query(const DevEphem* E, int n, const int* body, const double* t, double* out, int* status,
+                              hipStream_t s);
 
 }  // namespace assist_gpu

@@ -655,6 +655,64 @@ hipError_t launch_start_post(int n, const float* Y, const double* t0, const doub
 }
 
 // ------------------------------------------------------------------------------------------
+// Batched ephemeris queries (reference src/forces.c:175-263, assist_all_ephem, without its cache):
+// query q asks for ASSIST body body[q] at time t[q] (days from jd_ref) and receives
+// GM, x, y, z, vx, vy, vz, ax, ay, az in au, au/d, au/d^2; asteroids get the barycentric
+// position (heliocentric + the Sun's) and NaN velocity and acceleration, as in the reference.
+// One thread per query.
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) ephem_query_kernel(const DevEphem* __restrict__ Ep, int n,
+                                                          const int* __restrict__ body, const double* __restrict__ t,
+                                                          double* __restrict__ out, int* __restrict__ status) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const DevEphem& E = *Ep;
+    const int b = body[q];
+    const double tq = t[q];
+    double* o = out + (size_t)q * 10;
+    int st = kOk;
+    // ASSIST_ERROR_NEPHEM (4) for a negative body, ASSIST_ERROR_AST_FILE (2) for an asteroid without
+    // an asteroid file, ASSIST_ERROR_NAST (3) past the file's asteroids
+    if (b < 0) st = 4;
+    else if (b >= kNumPlanets && E.n_ast == 0) st = 2;
+    else if (b >= kNumPlanets + E.n_ast) st = 3;
+    else if (!(tq >= E.t_beg && tq <= E.t_end)) st = kCoverage;
+    if (st != kOk) {
+        for (int k = 0; k < 10; k++) o[k] = 0.0;
+        if (status) status[q] = st;
+        return;
+    }
+    double r[9];
+    if (b < kNumPlanets) {
+        const int s = b < 3 ? b : b + 1;  // body -> slot (the EMB's slot 3 is skipped)
+        const ChebOut<double> c = cheb_eval<double, true, true>(E.slot[s], tq);
+        r[0] = c.x; r[1] = c.y; r[2] = c.z; r[3] = c.vx; r[4] = c.vy; r[5] = c.vz; r[6] = c.ax; r[7] = c.ay; r[8] = c.az;
+        if (s == kSlotEarth || s == kSlotMoon) {
+            const ChebOut<double> e = cheb_eval<double, true, true>(E.slot[kSlotEMB], tq);
+            r[0] += e.x; r[1] += e.y; r[2] += e.z; r[3] += e.vx; r[4] += e.vy; r[5] += e.vz;
+            r[6] += e.ax; r[7] += e.ay; r[8] += e.az;
+        }
+        for (int k = 0; k < 9; k++) o[1 + k] = r[k] * E.inv_au;
+    } else {
+        const ChebOut<double> c = cheb_eval<double, false>(E.slot[kFirstAsteroidSlot + b - kNumPlanets], tq);
+        const ChebOut<double> sun = cheb_eval<double, false>(E.slot[0], tq);
+        o[1] = c.x * E.inv_au_ast + sun.x * E.inv_au;
+        o[2] = c.y * E.inv_au_ast + sun.y * E.inv_au;
+        o[3] = c.z * E.inv_au_ast + sun.z * E.inv_au;
+        for (int k = 4; k < 10; k++) o[k] = __longlong_as_double(0x7ff8000000000000LL);
+    }
+    o[0] = E.GM[b];
+    if (status) status[q] = kOk;
+}
+
+hipError_t launch_ephem_query(const DevEphem* E, int n, const int* body, const double* t, double* out, int* status,
+                              hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ephem_query_kernel, dim3((n + 255) / 256), dim3(256), 0, s, E, n, body, t, out, status);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
 // host launchers
 // ------------------------------------------------------------------------------------------
 template <int G, bool M, bool X = false>

@@ -276,6 +276,17 @@ int assist_batch_forces(const struct assist_ephem* ephem, const struct assist_ex
                         const double* t, const double* states, const double* params, double* acc, int* status,
                         int precision);
 
+/* Batched ephemeris queries on the GPU: query q is ASSIST body body[q] (0..26) at t[q] (days
+ * from jd_ref), as reference assist_all_ephem / assist_get_particle (src/forces.c:175-263,
+ * src/assist.c:530-554) answer one at a time.  out: n x 10 = GM, x, y, z, vx, vy, vz, ax, ay, az
+ * (au, au/d, au/d^2; asteroids: barycentric position, NaN velocity and acceleration); status:
+ * per query (ASSIST_ERROR_COVERAGE, ASSIST_ERROR_NEPHEM, ASSIST_ERROR_NAST, ASSIST_ERROR_AST_FILE;
+ * out is zero then) or NULL.  Host buffers here; device buffers and a stream below. */
+int assist_batch_get_particles(const struct assist_ephem* ephem, int n, const int* body, const double* t, double* out,
+                               int* status);
+int assist_batch_get_particles_device(const struct assist_ephem* ephem, int n, const int* d_body, const double* d_t,
+                                      double* d_out, int* d_status, void* stream);
+
 /* Integrates n systems with IAS15, system i from t0[i] to t1[i] (either direction), states
  * updated in place.  Host pointers; copies to and from the GPU are included. */
 int assist_batch_integrate(const struct assist_ephem* ephem, const struct assist_extras* opts,

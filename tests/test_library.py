@@ -259,3 +259,11 @@ def test_dense_output_checks_time_order_before_the_gpu(ephem):
                                               st.ctypes.data_as(C.POINTER(C.c_int)),
                                               steps.ctypes.data_as(C.POINTER(C.c_longlong)), None)
         assert rc == 100
+
+
+def test_batched_queries_fail_without_gpu(ephem):
+    from assist_amd import batch
+    if batch.gpu_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(batch.BatchError):
+        batch.get_particles(ephem, [0], [0.0])
