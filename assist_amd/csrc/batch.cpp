@@ -276,26 +276,47 @@ int lanes_for(int nvec, int requested) {
 }
 
 // Persistent workspace on the current device (grown as needed; one launch at a time).
+// Captured launches of the integration (set_counter + ias15_kernel), keyed by everything the
+// launch takes: the argument block (device pointers included) and the launch shape.
+struct GraphEntry {
+    IntegrateArgs args;
+    int G, threads, waves, blocks;
+    bool mixed;
+    hipGraphExec_t exec;
+};
+constexpr size_t kMaxGraphs = 8;
+
 struct Workspace {
     int device = -1;
     double* scratch = nullptr;
     size_t scratch_words = 0;
     int* queue = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_in = nullptr;
+    hipStream_t graph_stream = nullptr;  // default priority, non-blocking
+    std::vector<GraphEntry> graphs;      // most recently used last
 };
 Workspace g_ws;
+
+void clear_graphs() {
+    for (auto& e : g_ws.graphs) (void)hipGraphExecDestroy(e.exec);
+    g_ws.graphs.clear();
+}
 
 int ensure_workspace(size_t scratch_words) {
     int dev = 0;
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return ASSIST_BATCH_ERROR_DEVICE;
     if (g_ws.device != dev) {
+        g_ws.graphs.clear();  // (those of another device are not destroyed from this one)
         g_ws = Workspace();
         g_ws.device = dev;
         if (!hip_ok(hipMalloc(&g_ws.queue, sizeof(int)), "hipMalloc(queue)")) return ASSIST_BATCH_ERROR_DEVICE;
-        if (!hip_ok(hipEventCreate(&g_ws.ev0), "hipEventCreate") || !hip_ok(hipEventCreate(&g_ws.ev1), "hipEventCreate"))
+        if (!hip_ok(hipEventCreate(&g_ws.ev0), "hipEventCreate") || !hip_ok(hipEventCreate(&g_ws.ev1), "hipEventCreate") ||
+            !hip_ok(hipEventCreateWithFlags(&g_ws.ev_in, hipEventDisableTiming), "hipEventCreate") ||
+            !hip_ok(hipStreamCreateWithFlags(&g_ws.graph_stream, hipStreamNonBlocking), "hipStreamCreate"))
             return ASSIST_BATCH_ERROR_DEVICE;
     }
     if (scratch_words > g_ws.scratch_words) {
+        clear_graphs();  // captured launches hold the old scratch pointer
         if (g_ws.scratch) (void)hipFree(g_ws.scratch);
         g_ws.scratch = nullptr;
         g_ws.scratch_words = 0;
@@ -372,9 +393,58 @@ constexpr int kTuneMinSystems = 16384;  // smaller batches take the default shap
 constexpr int kTuneSample = 8192;       // systems per trial launch
 constexpr double kTuneSpan = 30.0;      // days integrated per trial system
 
+// Replays (capturing it first when new) the launch set_counter + ias15_kernel for `a` on the
+// library's graph stream, after the work already queued on `s`; *ms is the graph's HIP-event time.
+int run_ias15_graph(const IntegrateArgs& a, bool mixed, const LaunchShape& sh, int blocks, int total_groups,
+                    hipStream_t s, float* ms) {
+    GraphEntry* hit = nullptr;
+    for (auto& e : g_ws.graphs) {
+        if (e.G == sh.G && e.threads == sh.threads && e.waves == sh.waves && e.blocks == blocks && e.mixed == mixed &&
+            memcmp(&e.args, &a, sizeof a) == 0) {
+            hit = &e;
+            break;
+        }
+    }
+    hipStream_t gs = g_ws.graph_stream;
+    if (!hip_ok(hipEventRecord(g_ws.ev_in, s), "hipEventRecord") || !hip_ok(hipStreamWaitEvent(gs, g_ws.ev_in, 0), "hipStreamWaitEvent"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    if (!hit) {
+        if (g_ws.graphs.size() >= kMaxGraphs) {
+            (void)hipGraphExecDestroy(g_ws.graphs.front().exec);
+            g_ws.graphs.erase(g_ws.graphs.begin());
+        }
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (!hip_ok(hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+            return ASSIST_BATCH_ERROR_DEVICE;
+        const bool launched = hip_ok(launch_set_counter(g_ws.queue, total_groups, gs), "launch(set_counter)") &&
+                              hip_ok(launch_ias15(a, sh.G, mixed, sh.waves, blocks, sh.threads, gs, false), "launch(ias15)");
+        const bool captured = hip_ok(hipStreamEndCapture(gs, &graph), "hipStreamEndCapture");
+        if (!launched || !captured || !hip_ok(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate")) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return ASSIST_BATCH_ERROR_DEVICE;
+        }
+        (void)hipGraphDestroy(graph);
+        GraphEntry e;
+        memset(&e, 0, sizeof e);
+        e.args = a;
+        e.G = sh.G; e.threads = sh.threads; e.waves = sh.waves; e.blocks = blocks; e.mixed = mixed;
+        e.exec = exec;
+        g_ws.graphs.push_back(e);
+        hit = &g_ws.graphs.back();
+    }
+    if (!hip_ok(hipEventRecord(g_ws.ev0, gs), "hipEventRecord") || !hip_ok(hipGraphLaunch(hit->exec, gs), "hipGraphLaunch") ||
+        !hip_ok(hipEventRecord(g_ws.ev1, gs), "hipEventRecord") || !hip_ok(hipEventSynchronize(g_ws.ev1), "ias15 graph"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, g_ws.ev0, g_ws.ev1);
+    if (ms) *ms = t;
+    return ASSIST_SUCCESS;
+}
+
 // Launches set_counter + ias15_kernel on `s` and waits; *ms is the kernel's HIP-event time.
 int run_ias15(IntegrateArgs a, bool mixed, const LaunchShape& sh, int blocks_per_cu_cap, hipStream_t s, float* ms,
-              int* blocks_out) {
+              int* blocks_out, bool use_graph = false) {
     int dev = 0, ncu = 0, per_cu = 0;
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice") ||
         !hip_ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute") ||
@@ -390,6 +460,12 @@ int run_ias15(IntegrateArgs a, bool mixed, const LaunchShape& sh, int blocks_per
     if (rc) return rc;
     a.scratch = g_ws.scratch;
     a.queue = g_ws.queue;
+    if (use_graph) {
+        rc = run_ias15_graph(a, mixed, sh, blocks, total_groups, s, ms);
+        if (rc) return rc;
+        if (blocks_out) *blocks_out = blocks;
+        return ASSIST_SUCCESS;
+    }
     if (!hip_ok(launch_set_counter(g_ws.queue, total_groups, s), "launch(set_counter)") ||
         !hip_ok(hipEventRecord(g_ws.ev0, s), "hipEventRecord") ||
         !hip_ok(launch_ias15(a, sh.G, mixed, sh.waves, blocks, sh.threads, s), "launch(ias15)") ||
@@ -644,7 +720,7 @@ int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const str
     }
     float ms = 0.f;
     int blocks = 0;
-    rc = run_ias15(a, mixed, shape, cfg.blocks_per_cu, s, &ms, &blocks);
+    rc = run_ias15(a, mixed, shape, cfg.blocks_per_cu, s, &ms, &blocks, cfg.use_graph != 0);
     if (rc) return rc;
     const int G = shape.G, threads = shape.threads, waves = shape.waves;
     if (stats) {

@@ -780,12 +780,16 @@ size_t ias15_lds_bytes(int G, int nvec, int threads, int waves) {
            sizeof(double);
 }
 
-hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int waves, int blocks, int threads, hipStream_t s) {
+hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int waves, int blocks, int threads, hipStream_t s,
+                        bool set_attributes) {
     const void* f = ias15_lookup(G, mixed, waves, ext_model(a.fo));
     if (!f) return hipErrorInvalidValue;
     const size_t lds = ias15_lds_bytes(G, a.nvec, threads, waves);
-    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
+    hipError_t e = hipSuccess;
+    if (set_attributes) {  // (set beforehand by ias15_occupancy when the launch is being captured)
+        e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     void* params[] = {(void*)&a};
     e = hipLaunchKernel(f, dim3(blocks), dim3(threads), params, lds, s);
     if (e != hipSuccess) return e;

@@ -172,7 +172,8 @@ def run_form(form, args, cfg, ephem, bufs, n, nv, step_model):
     from assist_amd import batch, learned
     spec = FORMS[form]
     bc = batch.BatchConfig.default(precision=spec["precision"], lanes_per_system=args.lanes, block_size=args.block,
-                                   waves_per_simd=args.waves, min_dt=cfg["min_dt"], max_steps=200000)
+                                   waves_per_simd=args.waves, min_dt=cfg["min_dt"], max_steps=200000,
+                                   use_graph=1 if args.launch == "graph" else 0)
     d_t0, d_t1, d_init, d_states, d_params, d_status, d_steps, d_evals = bufs
 
     def one_step():
@@ -214,6 +215,8 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0, help="kernel variant: 1 or 2 waves per SIMD (0 = choose)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--launch", default="graph", choices=["graph", "direct"],
+                    help="integration launches: a HIP graph captured in the warmup and replayed, or direct")
     ap.add_argument("--copy-kernel-mib", type=int, default=0,
                     help="also copy this many MiB device-to-device before the timed region, to calibrate the "
                          "FETCH_SIZE/WRITE_SIZE counters")
@@ -280,7 +283,8 @@ def main():
         "data": "synthetic ephemeris (seed 0) and synthetic particle states",
         "config": {"workload": cfg["workload"], "systems": n, "vectors_per_system": nv, "form": args.form,
                    "particle_days_per_step": pdays, "lanes_per_system": st.lanes_per_system,
-                   "block_size": st.block_size, "grid_blocks": st.grid_blocks, "waves_per_simd": st.waves_per_simd},
+                   "block_size": st.block_size, "grid_blocks": st.grid_blocks, "waves_per_simd": st.waves_per_simd,
+                   "launch": args.launch},
         "plain_fp64": {"value": plain_value, "ms_per_step": plain["elapsed"] / args.steps * 1e3,
                        "kernel_ms": plain["kernel_ms"], "evals_total": int(plain["evals"].sum())},
         "speedup_vs_plain_fp64": value / plain_value,

@@ -251,7 +251,9 @@ struct assist_batch_config {
     int block_size;       /* 0 = choose, else a multiple of 64 up to 256 */
     int blocks_per_cu;    /* 0 = occupancy limit */
     int waves_per_simd;   /* 0 = choose (autotuned), else 1 or 2: the kernel variant's register budget */
-    int reserved_;
+    int use_graph;        /* 1: launch the integration as a captured HIP graph, captured once per
+                             argument set and replayed (the launches of a replay take the same
+                             device pointers); 0 (default): direct launches */
 };
 
 struct assist_batch_stats {
