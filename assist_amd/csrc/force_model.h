@@ -775,10 +775,23 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
     const bool eih = (forces & F_GR_EIH) && O.gr_eih_sources > 0;
     const bool want_sun_vel = eih || ((forces & F_NONGRAV) && ng) || (forces & F_GR_SIMPLE);
 
+    // kStream (one lane per system): the slots are evaluated one at a time in step 3 rather than
+    // all first, which would keep 28 positions live through the per-body work
+    constexpr bool kStream = (G == 1 && !Ext && std::is_same<D, double>::value);
+    D sun[3], sunv[3], emb[3], earth[3];
+    const D inv_au = D(E.inv_au);
+    D px[kStream ? 1 : NR][3];
+    D pv[NRV][3], pa[NRV][3];
+    if constexpr (kStream) {
+        const ChebOut<D> cs = cheb_eval<D, true>(slots[0], t);
+        sun[0] = cs.x * inv_au; sun[1] = cs.y * inv_au; sun[2] = cs.z * inv_au;
+        sunv[0] = cs.vx * inv_au; sunv[1] = cs.vy * inv_au; sunv[2] = cs.vz * inv_au;
+        const ChebOut<D> ce = cheb_eval<D, false>(slots[kSlotEMB], t);
+        emb[0] = ce.x; emb[1] = ce.y; emb[2] = ce.z;
+        earth[0] = earth[1] = earth[2] = D(0.0);  // set in step 3
+    } else {
     // 1. Chebyshev evaluations of the slots owned by this lane; round 0 (which holds the Sun,
     //    slot 0, on lane 0) also evaluates velocities when the Sun's velocity is needed
-    D px[NR][3];
-    D pv[NRV][3], pa[NRV][3];
     D sunv_l[3] = {D(0.0), D(0.0), D(0.0)};
 #pragma unroll
     for (int r = 0; r < NR; r++) {
@@ -808,8 +821,6 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
         }
     }
     // 2. exchange Sun (au) and EMB (km); compose Earth, Moon and asteroids
-    D sun[3], sunv[3], emb[3];
-    const D inv_au = D(E.inv_au);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         sun[c] = gbcast<G, 0>(px[0][c] * inv_au);
@@ -844,9 +855,9 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
             for (int c = 0; c < 3; c++) px[r][c] = fma(px[r][c], D(E.inv_au_ast), sun[c]);
         }
     }
-    D earth[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) earth[c] = gbcast<G, kSlotEarth % G>(px[kSlotEarth / G][c]);
+    }
     // frame origin: the Earth's barycentric state when geocentric, else zero
     D xo[3] = {D(0.0), D(0.0), D(0.0)}, vo[3] = {D(0.0), D(0.0), D(0.0)}, ao[3] = {D(0.0), D(0.0), D(0.0)};
     if constexpr (Ext) {
@@ -872,18 +883,14 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
     TS jd[6] = {TS(0), TS(0), TS(0), TS(0), TS(0), TS(0)};  // xx yy zz xy yz xz
     D S0 = D(0.0), S1 = D(0.0), dS0[3] = {D(0.0), D(0.0), D(0.0)}, aj[3] = {D(0.0), D(0.0), D(0.0)};
     const int nbody = kNumPlanets + E.n_ast;
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-        const int s = r * G + lane;
-        if (s >= kNumSlots || s == kSlotEMB) continue;
-        const int b = slot_body(s);
-        if (b >= nbody) continue;
+    // body b at barycentric position pk (au)
+    auto body_term = [&](int b, const D pk[3]) {
         const D GM = D(gm[b]);
         const bool allowed = (b == 0) ? (forces & F_SUN) : (b < kNumPlanets ? (forces & F_PLANETS) : (forces & F_ASTEROIDS));
         // (with the extended model, xo is the frame origin: x - (xk - xo) == x + (xo - xk))
-        const D dx = Ext ? xr[0] - (px[r][0] - xo[0]) : xr[0] - px[r][0];
-        const D dy = Ext ? xr[1] - (px[r][1] - xo[1]) : xr[1] - px[r][1];
-        const D dz = Ext ? xr[2] - (px[r][2] - xo[2]) : xr[2] - px[r][2];
+        const D dx = Ext ? xr[0] - (pk[0] - xo[0]) : xr[0] - pk[0];
+        const D dy = Ext ? xr[1] - (pk[1] - xo[1]) : xr[1] - pk[1];
+        const D dz = Ext ? xr[2] - (pk[2] - xo[2]) : xr[2] - pk[2];
         // Mixed form: an asteroid's point mass runs in FP32 unless the particle is within
         // kFp32MinDist of it, where the FP32 round-off of the term would reach IAS15's
         // error estimate (see kFp32MinDist).
@@ -901,7 +908,7 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
                 jd[0] += g * (fx * fx * r5 - r3inv); jd[1] += g * (fy * fy * r5 - r3inv); jd[2] += g * (fz * fz * r5 - r3inv);
                 jd[3] += g * (fx * fy * r5); jd[4] += g * (fy * fz * r5); jd[5] += g * (fx * fz * r5);
             }
-            continue;
+            return;
         }
         const D r2 = dx * dx + dy * dy + dz * dz;
         const D ir = inv_sqrt(r2);
@@ -921,13 +928,46 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
             S0 += GM * ir;
             dS0[0] += ax; dS0[1] += ay; dS0[2] += az;
             if (!Ext && b != 0) {
-                const D sx = sun[0] - px[r][0], sy = sun[1] - px[r][1], sz = sun[2] - px[r][2];
+                const D sx = sun[0] - pk[0], sy = sun[1] - pk[1], sz = sun[2] - pk[2];
                 const D q2 = sx * sx + sy * sy + sz * sz;
                 const D iq = inv_sqrt(q2);
                 S1 += GM * iq;
                 const D f = GM * (iq * iq * iq);
                 aj[0] -= f * sx; aj[1] -= f * sy; aj[2] -= f * sz;
             }
+        }
+    };
+    if constexpr (kStream) {
+        // one system per lane: each slot is evaluated and folded in at once (the Sun and the
+        // EMB, which the others need, were evaluated above)
+#pragma unroll 2
+        for (int s = 1; s < kNumSlots; s++) {
+            if (s == kSlotEMB) continue;
+            const ChebOut<D> c = cheb_eval<D, false>(slots[s], t);
+            D pk[3];
+            if (s < kFirstAsteroidSlot) {
+                const bool em = (s == kSlotEarth || s == kSlotMoon);
+                pk[0] = (em ? c.x + emb[0] : c.x) * inv_au;
+                pk[1] = (em ? c.y + emb[1] : c.y) * inv_au;
+                pk[2] = (em ? c.z + emb[2] : c.z) * inv_au;
+                if (s == kSlotEarth) { earth[0] = pk[0]; earth[1] = pk[1]; earth[2] = pk[2]; }
+            } else {
+                pk[0] = fma(c.x, D(E.inv_au_ast), sun[0]);
+                pk[1] = fma(c.y, D(E.inv_au_ast), sun[1]);
+                pk[2] = fma(c.z, D(E.inv_au_ast), sun[2]);
+            }
+            const int b = slot_body(s);
+            if (b < nbody) body_term(b, pk);
+        }
+        body_term(0, sun);
+    } else {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int s = r * G + lane;
+            if (s >= kNumSlots || s == kSlotEMB) continue;
+            const int b = slot_body(s);
+            if (b >= nbody) continue;
+            body_term(b, px[r]);
         }
     }
     // 4. group sums

@@ -37,6 +37,8 @@ CONFIGS = {
                         "min_dt 0.001 d, forces 0x7F"),
     "c2": dict(n=4096, n_var=6, span=3650.0, neo=0.0, seed=2, uneven=False, min_dt=0.0,
                workload="4096 main-belt systems x (1 + 6 variational), 3650 d, eps 1e-9, forces 0x7F"),
+    "c5": dict(n=1048576, n_var=0, span=365.0, neo=0.3, seed=5, uneven=False, min_dt=0.001,
+               workload="1048576 systems (70% main-belt, 30% NEO), 365 d, eps 1e-9, min_dt 0.001 d, forces 0x7F"),
     "c3": dict(n=65536, n_var=0, span=None, neo=0.3, seed=3, uneven=True, min_dt=0.001,
                workload="65536 systems (70% main-belt, 30% NEO), log-uniform 30-10000 d spans, half backward, "
                         "10% Marsden, eps 1e-9, min_dt 0.001 d"),
