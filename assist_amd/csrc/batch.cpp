@@ -294,6 +294,13 @@ struct Workspace {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_in = nullptr;
     hipStream_t graph_stream = nullptr;  // default priority, non-blocking
     std::vector<GraphEntry> graphs;      // most recently used last
+    // queue order of uneven batches (keys, sorted keys, indices, order, sort temporaries)
+    unsigned long long* okeys = nullptr;
+    unsigned long long* okeys_alt = nullptr;
+    int* ovals = nullptr;
+    int* order = nullptr;
+    void* otemp = nullptr;
+    size_t order_n = 0, otemp_bytes = 0;
 };
 Workspace g_ws;
 
@@ -323,6 +330,28 @@ int ensure_workspace(size_t scratch_words) {
         if (!hip_ok(hipMalloc(&g_ws.scratch, scratch_words * sizeof(double)), "hipMalloc(scratch)")) return ASSIST_BATCH_ERROR_DEVICE;
         g_ws.scratch_words = scratch_words;
     }
+    return ASSIST_SUCCESS;
+}
+
+// Buffers of the span sort for n systems (the workspace of the current device must exist).
+int ensure_order(int n) {
+    if ((size_t)n <= g_ws.order_n) return ASSIST_SUCCESS;
+    clear_graphs();  // captured launches hold the old order pointer
+    (void)hipFree(g_ws.okeys); (void)hipFree(g_ws.okeys_alt); (void)hipFree(g_ws.ovals); (void)hipFree(g_ws.order);
+    (void)hipFree(g_ws.otemp);
+    g_ws.okeys = g_ws.okeys_alt = nullptr;
+    g_ws.ovals = g_ws.order = nullptr;
+    g_ws.otemp = nullptr;
+    g_ws.order_n = g_ws.otemp_bytes = 0;
+    const size_t tb = span_order_temp_bytes(n);
+    if (!hip_ok(hipMalloc(&g_ws.okeys, (size_t)n * sizeof(unsigned long long)), "hipMalloc(order)") ||
+        !hip_ok(hipMalloc(&g_ws.okeys_alt, (size_t)n * sizeof(unsigned long long)), "hipMalloc(order)") ||
+        !hip_ok(hipMalloc(&g_ws.ovals, (size_t)n * sizeof(int)), "hipMalloc(order)") ||
+        !hip_ok(hipMalloc(&g_ws.order, (size_t)n * sizeof(int)), "hipMalloc(order)") ||
+        !hip_ok(hipMalloc(&g_ws.otemp, tb > 0 ? tb : 1), "hipMalloc(order)"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    g_ws.order_n = (size_t)n;
+    g_ws.otemp_bytes = tb;
     return ASSIST_SUCCESS;
 }
 
@@ -725,6 +754,17 @@ int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const str
     if (tune) {
         rc = tuned_shape(a, mixed, s, &shape);
         if (rc) return rc;
+    }
+    // uneven batches: hand out the longest spans first (the order is identity for equal spans)
+    if (n >= kTuneMinSystems && !dense && d_t1) {
+        rc = ensure_workspace(0);
+        if (rc) return rc;
+        rc = ensure_order(n);
+        if (rc) return rc;
+        if (!hip_ok(launch_span_order(n, d_t0, d_t1, g_ws.okeys, g_ws.okeys_alt, g_ws.ovals, g_ws.order, g_ws.otemp,
+                                      g_ws.otemp_bytes, s), "launch(span order)"))
+            return ASSIST_BATCH_ERROR_DEVICE;
+        a.order = g_ws.order;
     }
     float ms = 0.f;
     int blocks = 0;

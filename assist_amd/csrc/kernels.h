@@ -65,6 +65,8 @@ struct IntegrateArgs {
     const int* out_off;
     const double* out_t;
     double* out_states;
+    // queue order (optional): the q-th system handed out is order[q] (longest spans first)
+    const int* order;
 };
 
 // Trace record: t, x0[3], v0[3] at the start of the step, the previous step, the accepted
@@ -93,6 +95,12 @@ hipError_t ias15_occupancy(int G, bool mixed, int waves, bool ext, int nvec, int
 inline bool ext_model(const ForceOpts& f) { return f.geocentric != 0 || ((f.forces & 0x40) && f.gr_eih_sources > 1); }
 size_t ias15_lds_bytes(int G, int nvec, int threads, int waves);
 hipError_t launch_set_counter(int* q, int v, hipStream_t s);
+// Queue order for uneven batches: order[0..n) = system indices sorted by |t1 - t0|, longest first
+// (stable, so equal spans keep their index order).  Buffers: keys/keys_alt n words, vals n ints.
+size_t span_order_temp_bytes(int n);
+hipError_t launch_span_order(int n, const double* t0, const double* t1, unsigned long long* keys,
+                             unsigned long long* keys_alt, int* vals, int* order, void* temp, size_t temp_bytes,
+                             hipStream_t s);
 hipError_t launch_ephem_query(const DevEphem* E, int n, const int* body, const double* t, double* out, int* status,
                               hipStream_t s);
 

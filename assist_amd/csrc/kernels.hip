@@ -23,6 +23,8 @@
 #include "ias15_tables.h"
 #include "kernels.h"
 
+#include <hipcub/hipcub.hpp>
+
 namespace assist_gpu {
 
 __constant__ double cH[8] = IAS15_H_INIT;
@@ -174,8 +176,9 @@ ias15_kernel(IntegrateArgs args) {
     int phase = 0, n = 1, it = 0, last = 0, status = 0, attempts = 0;
     long long steps = 0, evals = 0, rejected = 0;
     bool ng = false;
-    int sys = grp;
-    bool active = sys < ka->nsys;
+    // the queue hands out positions q; order[q] (when given) is the system at that position
+    bool active = grp < ka->nsys;
+    int sys = active ? (ka->order ? ka->order[grp] : grp) : 0;
     int cur = 0, cend = 0;  // dense output: next and end index of this system's output times
 
     auto load = [&](int s) {
@@ -230,9 +233,11 @@ ias15_kernel(IntegrateArgs args) {
         int next = 0;
         if (lane == 0) next = atomicAdd(ka->queue, 1);
         next = gbcast_i<G, 0>(next);
-        sys = next;
-        active = sys < ka->nsys;
-        if (active) load(sys);
+        active = next < ka->nsys;
+        if (active) {
+            sys = ka->order ? ka->order[next] : next;
+            load(sys);
+        }
     };
     // Prepare an attempt of the current step: trim to t_end, predict b from the last
     // accepted step, convert to g.  Returns false when the step cannot be taken.
@@ -710,6 +715,38 @@ hipError_t launch_ephem_query(const DevEphem* E, int n, const int* body, const d
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(ephem_query_kernel, dim3((n + 255) / 256), dim3(256), 0, s, E, n, body, t, out, status);
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// Queue order of uneven batches: systems sorted by span, longest first, so that the persistent
+// kernel starts the long integrations early and its tail is not set by one started late.
+// ------------------------------------------------------------------------------------------
+__global__ void span_keys_kernel(int n, const double* __restrict__ t0, const double* __restrict__ t1,
+                                 unsigned long long* __restrict__ keys, int* __restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double span = fabs(t1[i] - t0[i]);
+    // non-negative doubles sort like their bit patterns; NaN sorts last
+    keys[i] = (span == span) ? (unsigned long long)__double_as_longlong(span) : 0ull;
+    vals[i] = i;
+}
+
+size_t span_order_temp_bytes(int n) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, (const unsigned long long*)nullptr,
+                                                       (unsigned long long*)nullptr, (const int*)nullptr,
+                                                       (int*)nullptr, n, 0, 64);
+    return bytes;
+}
+
+hipError_t launch_span_order(int n, const double* t0, const double* t1, unsigned long long* keys,
+                             unsigned long long* keys_alt, int* vals, int* order, void* temp, size_t temp_bytes,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(span_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, t0, t1, keys, vals);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys, keys_alt, vals, order, n, 0, 64, s);
 }
 
 // ------------------------------------------------------------------------------------------

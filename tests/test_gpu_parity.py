@@ -263,3 +263,20 @@ def test_integrate_one_lane_per_system(gpu, ephem, oracle, model, lanes, block):
     print("1 lane per system, 10 yr: max |dr| = %.3e m" % dpos.max())
     assert dpos.max() <= 1.0
     assert np.abs(r.steps - o["steps"]).max() <= 2
+
+
+def test_uneven_batch_queue_order_keeps_results(gpu, ephem, model):
+    """Batches of 16 384 systems or more hand out the longest spans first; every system's result
+    is the one it gets in a small (unordered) batch, bit for bit."""
+    from assist_amd import batch, synthetic
+    rng = np.random.default_rng(121)
+    n = 20000
+    states, _, _ = population(model, n, seed=121, n_var=6)
+    span = synthetic.log_uniform_spans(n, rng, 1.0, 100.0)
+    cfg = batch.BatchConfig.default(min_dt=0.001)
+    big = batch.integrate(ephem, 0.0, span, states, config=cfg)
+    for lo in (0, 10000):
+        sl = slice(lo, lo + 10000)
+        small = batch.integrate(ephem, 0.0, span[sl], states[sl], config=cfg)
+        assert np.array_equal(small.states, big.states[sl])
+        assert (small.steps == big.steps[sl]).all() and (small.status == big.status[sl]).all()
