@@ -475,7 +475,7 @@ int run_ias15_graph(const IntegrateArgs& a, bool mixed, const LaunchShape& sh, i
 
 // Launches set_counter + ias15_kernel on `s` and waits; *ms is the kernel's HIP-event time.
 int run_ias15(IntegrateArgs a, bool mixed, const LaunchShape& sh, int blocks_per_cu_cap, hipStream_t s, float* ms,
-              int* blocks_out, bool use_graph = false) {
+              int* blocks_out, bool use_graph = false, bool use_cache = true) {
     int dev = 0, ncu = 0, per_cu = 0;
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice") ||
         !hip_ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute") ||
@@ -487,9 +487,12 @@ int run_ias15(IntegrateArgs a, bool mixed, const LaunchShape& sh, int blocks_per
     const long long max_blocks = (long long)per_cu * ncu;
     const int blocks = (int)(need_blocks < max_blocks ? need_blocks : max_blocks);
     const int total_groups = blocks * sh.threads / sh.G;
-    int rc = ensure_workspace((size_t)blocks * sh.threads * 42);
+    // scratch: the b/e records of every lane (42 doubles), then the substep ephemeris cache
+    const int cw = use_cache ? ias15_cache_words(sh.G, ext_model(a.fo)) : 0;
+    int rc = ensure_workspace((size_t)blocks * sh.threads * (42 + cw));
     if (rc) return rc;
     a.scratch = g_ws.scratch;
+    a.ecache = cw ? g_ws.scratch + (size_t)blocks * sh.threads * 42 : nullptr;
     a.queue = g_ws.queue;
     if (use_graph) {
         rc = run_ias15_graph(a, mixed, sh, blocks, total_groups, s, ms);
@@ -768,7 +771,7 @@ int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const str
     }
     float ms = 0.f;
     int blocks = 0;
-    rc = run_ias15(a, mixed, shape, cfg.blocks_per_cu, s, &ms, &blocks, cfg.use_graph != 0);
+    rc = run_ias15(a, mixed, shape, cfg.blocks_per_cu, s, &ms, &blocks, cfg.use_graph != 0, cfg.substep_cache_off == 0);
     if (rc) return rc;
     const int G = shape.G, threads = shape.threads, waves = shape.waves;
     if (stats) {

@@ -276,6 +276,150 @@ AHD ChebOut<D> cheb_eval(const DevSlot& S, D t) {
     return o;
 }
 
+// ---- substep ephemeris cache and wave-wide Chebyshev rounds (8-lane groups) ---------------
+// The reference keeps up to 7 ephemeris results per body keyed by the exact time (reference
+// src/forces.c:175-263), which IAS15's substep times hit: every predictor-corrector iteration of
+// a step evaluates the forces at the same 7 times.  The integration kernel does the same per
+// lane group: the first iteration of a step attempt stores the raw Chebyshev positions of each
+// substep in the group's cache (kEcFill) and the later iterations load them (kEcUse).  On a trip
+// where some groups of a wave need fresh positions (kEcEval: the step's first evaluation, or
+// kEcFill), their 28 evaluations each are spread slot-major over all active lanes of the wave, so
+// the wave runs ceil(28 k / A) rounds for k such groups and A active lanes instead of 4, and the
+// results reach the owning lanes through ds_bpermute.  Every value comes from the same code and
+// inputs as in the per-group evaluation, so results are unchanged bit for bit.
+enum : int { kEcEval = 0, kEcFill = 1, kEcUse = 2 };
+constexpr int kEcWords = 88;  // per group and substep: raw positions (km) of the 28 slots, then
+                              // the Sun's raw velocity (words 84-86), one pad word
+// Rows of a group in a wave's LDS exchange area: the 28 slots' positions, then the Sun's velocity.
+constexpr int kXchRows = kNumSlots + 1;
+struct SubstepCache {
+    double* base;             // this wave's region: [8 groups][7 substeps][kEcWords] doubles, or
+                              // nullptr (no cache: every trip evaluates, mode kEcEval)
+    double* xch;              // this wave's LDS exchange area: [8 groups][kXchRows][3] doubles
+    int n;                    // substep 1..7 (kEcFill)
+    int mode;
+    double px[4][3], sv[3];   // kEcUse: this lane's positions and the Sun's velocity, loaded by
+                              // ec_prefetch at the end of the previous trip
+};
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// Index of the i-th group (0..7) whose leader bit (bit 8 g) is set in `leaders` (uniform).
+__device__ __forceinline__ int nth_group(unsigned long long leaders, int i) {
+    int g = 0, cnt = 0;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const bool b = ((leaders >> (8 * t)) & 1ULL) != 0;
+        g = (b && cnt == i) ? t : g;
+        cnt += b ? 1 : 0;
+    }
+    return g;
+}
+
+// Loads this lane's cached positions of substep n (and the Sun's velocity) into sc for the next
+// trip, so that their latency overlaps the rest of the current trip.
+__device__ __forceinline__ void ec_prefetch(SubstepCache& sc, int lane, int n) {
+    // the cache stores of the wave's earlier trips complete before these loads
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    const double* p = sc.base + ((size_t)(__lane_id() >> 3) * 7 + (n - 1)) * kEcWords;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int s = r * 8 + lane;
+        const int sl = s < kNumSlots ? s : 0;  // (lanes past the last slot read slot 0)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double v = __builtin_nontemporal_load(p + 3 * sl + c);
+            sc.px[r][c] = (s < kNumSlots) ? v : 0.0;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) sc.sv[c] = __builtin_nontemporal_load(p + 84 + c);
+}
+
+// Raw positions (km) of the slots r * 8 + lane (r < 4) of this lane's group and, on the group's
+// lane 0, the Sun's raw velocity, for 8-lane groups (see the cache description above).  All
+// active lanes of the wave call this together; whole groups are active or not.  The evaluating
+// lanes write each result to the wave's LDS exchange area, from which the owners read it.
+__device__ __forceinline__ void wave_positions(const DevSlot* slots, int lane, double t, bool want_vel,
+                                               const SubstepCache& sc, double px[4][3], double sv[3]) {
+    constexpr unsigned long long kLeaders = 0x0101010101010101ULL;
+    const int wl = __lane_id();
+    const int grp = wl >> 3;
+    const unsigned long long below = (1ULL << (8 * grp)) - 1ULL;
+    const unsigned long long act = __ballot(1) & kLeaders;        // active groups
+    const bool need = sc.mode != kEcUse;
+    const unsigned long long nd = __ballot(need) & kLeaders;      // groups to evaluate
+    const bool any_vel = __ballot(need && want_vel) != 0ULL;
+    const int k = __popcll(nd);
+    const int A = 8 * __popcll(act);                               // active lanes
+    const int a = 8 * __popcll(act & below) + lane;                // this lane's active rank
+    // rounds: item j = s k + i (slot s of the i-th group to evaluate) on the lane of active rank
+    // j mod A in round j / A; the Sun's items (s = 0) fall in round 0, which evaluates velocities
+    const int items = kNumSlots * k;
+    const int R = (items + A - 1) / A;
+    const float inv_k = k > 0 ? 1.0f / (float)k : 0.0f;
+    double* const x = sc.xch;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (q >= R) break;
+        const int j = q * A + a;
+        const int s = (int)(((float)j + 0.5f) * inv_k);
+        const int i = j - s * k;
+        const int gi = nth_group(nd, i < k ? i : 0);
+        const double ti = __shfl(t, 8 * gi, 64);
+        const int nm = __shfl(sc.n * 4 + sc.mode, 8 * gi, 64);
+        if (j < items) {
+            ChebOut<double> c;
+            if (q == 0 && any_vel) c = cheb_eval<double, true>(slots[s], ti);
+            else c = cheb_eval<double, false>(slots[s], ti);
+            double* e = x + (gi * kXchRows + s) * 3;
+            e[0] = c.x; e[1] = c.y; e[2] = c.z;
+            if (s == 0) {
+                double* ev = x + (gi * kXchRows + kNumSlots) * 3;
+                ev[0] = c.vx; ev[1] = c.vy; ev[2] = c.vz;
+            }
+            if ((nm & 3) == kEcFill) {
+                double* p = sc.base + ((size_t)gi * 7 + ((nm >> 2) - 1)) * kEcWords;
+                __builtin_nontemporal_store(c.x, p + 3 * s);
+                __builtin_nontemporal_store(c.y, p + 3 * s + 1);
+                __builtin_nontemporal_store(c.z, p + 3 * s + 2);
+                if (s == 0) {
+                    __builtin_nontemporal_store(c.vx, p + 84);
+                    __builtin_nontemporal_store(c.vy, p + 85);
+                    __builtin_nontemporal_store(c.vz, p + 86);
+                }
+            }
+        }
+    }
+    // (LDS operations of one wave complete in order; the fence keeps the compiler's order too)
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (need) {
+        const double* e = x + grp * kXchRows * 3;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int s = r * 8 + lane;
+            const int sl = s < kNumSlots ? s : 0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double v = e[3 * sl + c];
+                px[r][c] = (s < kNumSlots) ? v : 0.0;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) sv[c] = e[3 * (kXchRows - 1) + c];
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) px[r][c] = sc.px[r][c];
+#pragma unroll
+        for (int c = 0; c < 3; c++) sv[c] = sc.sv[c];
+    }
+}
+#elif defined(__HIPCC__)
+// (the host pass of hipcc only type-checks the kernels that call it)
+__device__ void ec_prefetch(SubstepCache& sc, int lane, int n);
+#endif
+
 // ---- the 3x9 Jacobian of the real acceleration --------------------------------------------
 template <typename T>
 struct Jac {
@@ -764,7 +908,7 @@ AHD void eih_sources(const DevEphem& E, const ForceOpts& O, const double* gm, in
 template <int G, bool Mixed, typename D, bool Ext = false>
 AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, const ForceOpts& O, int lane, D t, const D xr[3],
                     const D vr[3], const double A[3], bool ng, bool need_jac, EvalResult<D>& out,
-                    Jac<typename TermScalar<D, Mixed>::type>& J) {
+                    Jac<typename TermScalar<D, Mixed>::type>& J, const SubstepCache* sc = nullptr) {
     using TS = typename TermScalar<D, Mixed>::type;
     static_assert(!Ext || (!Mixed && std::is_same<D, double>::value), "the extended model is FP64 only");
     constexpr int NR = (kNumSlots + G - 1) / G;
@@ -791,10 +935,21 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
         earth[0] = earth[1] = earth[2] = D(0.0);  // set in step 3
     } else {
     // 1. Chebyshev evaluations of the slots owned by this lane; round 0 (which holds the Sun,
-    //    slot 0, on lane 0) also evaluates velocities when the Sun's velocity is needed
+    //    slot 0, on lane 0) also evaluates velocities when the Sun's velocity is needed.  With a
+    //    substep cache (8-lane groups), the wave-wide rounds or the cache provide them instead.
     D sunv_l[3] = {D(0.0), D(0.0), D(0.0)};
+    bool from_wave = false;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (G == 8 && !Ext && std::is_same<D, double>::value) {
+        if (sc) {
+            wave_positions(slots, lane, t, want_sun_vel, *sc, px, sunv_l);
+            from_wave = true;
+        }
+    }
+#endif
 #pragma unroll
     for (int r = 0; r < NR; r++) {
+        if (from_wave) break;
         const int s = r * G + lane;
         if (s < kNumSlots) {
             ChebOut<D> c;

@@ -67,7 +67,20 @@ struct IntegrateArgs {
     double* out_states;
     // queue order (optional): the q-th system handed out is order[q] (longest spans first)
     const int* order;
+    // substep ephemeris cache of 8-lane groups (optional): [waves][8][7][kEcWords] doubles
+    double* ecache;
 };
+
+// Doubles of the substep ephemeris cache per launched thread (0 when the kernel keeps none):
+// 8-lane groups outside the extended model keep kEcWords per substep and group.
+__host__ __device__ constexpr int ias15_cache_words(int G, bool ext) {
+    return (G == 8 && !ext) ? 7 * 88 / 8 : 0;
+}
+// Doubles of LDS per wave for the exchange of the wave-wide Chebyshev rounds (8-lane groups
+// outside the extended model): 8 groups x 29 rows (28 slots, the Sun's velocity) x 3.
+__host__ __device__ constexpr int ias15_xch_words(int G, bool ext) {
+    return (G == 8 && !ext) ? 8 * 29 * 3 : 0;
+}
 
 // Trace record: t, x0[3], v0[3] at the start of the step, the previous step, the accepted
 // step, IAS15's proposal for the next step before the growth cap, a0[3], b[7][3] (k-major).
@@ -93,7 +106,7 @@ hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int waves, in
 hipError_t ias15_occupancy(int G, bool mixed, int waves, bool ext, int nvec, int threads, int* blocks_per_cu);
 // The extended force model (geocentric frame or several EIH sources) has kernels of its own.
 inline bool ext_model(const ForceOpts& f) { return f.geocentric != 0 || ((f.forces & 0x40) && f.gr_eih_sources > 1); }
-size_t ias15_lds_bytes(int G, int nvec, int threads, int waves);
+size_t ias15_lds_bytes(int G, int nvec, int threads, int waves, bool ext);
 hipError_t launch_set_counter(int* q, int v, hipStream_t s);
 // Queue order for uneven batches: order[0..n) = system indices sorted by |t1 - t0|, longest first
 // (stable, so equal spans keep their index order).  Buffers: keys/keys_alt n words, vals n ints.

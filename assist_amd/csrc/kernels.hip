@@ -171,6 +171,9 @@ ias15_kernel(IntegrateArgs args) {
     const int col = (own || cols_per_group == G) ? (int)(threadIdx.x / G) * cols_per_group + lane
                                                  : (int)(blockDim.x / G) * cols_per_group;
     const LdsArr b{s_cols + (size_t)col * kLdsColWords}, g{s_cols + (size_t)col * kLdsColWords + 21};
+    // this wave's exchange area of the wave-wide Chebyshev rounds, after the columns
+    double* const s_xch = s_cols + ((size_t)(blockDim.x / G) * cols_per_group + 1) * kLdsColWords +
+                          (size_t)(threadIdx.x >> 6) * ias15_xch_words(G, Ext);
     double A[3], dA[3];
     double t = 0, t_end = 0, dt = 0, dts = 0, dt_last = 0, pc_err_last = 0;
     int phase = 0, n = 1, it = 0, last = 0, status = 0, attempts = 0;
@@ -180,6 +183,17 @@ ias15_kernel(IntegrateArgs args) {
     bool active = grp < ka->nsys;
     int sys = active ? (ka->order ? ka->order[grp] : grp) : 0;
     int cur = 0, cend = 0;  // dense output: next and end index of this system's output times
+    constexpr bool kWaveRounds = ias15_cache_words(G, Ext) > 0;
+    SubstepCache scache;
+    // the next trip is a later iteration at substep nn: load its cached positions now
+    auto prefetch = [&](int nn) {
+        if constexpr (kWaveRounds) {
+            if (ka->ecache) {
+                scache.base = ka->ecache + (size_t)(tid >> 6) * (64 * ias15_cache_words(G, Ext));
+                ec_prefetch(scache, lane, nn);
+            }
+        }
+    };
 
     auto load = [&](int s) {
         const double* st = ka->states + ((size_t)s * nv + (own ? lane : 0)) * 6;
@@ -344,7 +358,18 @@ ias15_kernel(IntegrateArgs args) {
         EvalResult<double> R;
         Jac<TS> J;
         const ForceOpts fo = ka->fo;
-        force_eval<G, Mixed, double, Ext>(E, s_slots, s_gm, fo, lane, te, xr, vr, A, ng, nv > 1, R, J);
+        // 8-lane groups: wave-wide Chebyshev rounds and the substep ephemeris cache (the first
+        // iteration of an attempt stores the positions of its substeps, the later iterations,
+        // at the same substep times, use the ones ec_prefetch loaded at the end of the last trip)
+        const SubstepCache* scp = nullptr;
+        if constexpr (kWaveRounds) {
+            scache.base = ka->ecache ? ka->ecache + (size_t)(tid >> 6) * (64 * ias15_cache_words(G, Ext)) : nullptr;
+            scache.xch = s_xch;
+            scache.n = n;
+            scache.mode = (!ka->ecache || phase == 0) ? kEcEval : (it == 1 ? kEcFill : kEcUse);
+            scp = &scache;
+        }
+        force_eval<G, Mixed, double, Ext>(E, s_slots, s_gm, fo, lane, te, xr, vr, A, ng, nv > 1, R, J, scp);
         double at[3];
         own_accel<double, TS>(lane, R, J, xe, ve, dA, ng, at);
         evals++;
@@ -400,13 +425,18 @@ ias15_kernel(IntegrateArgs args) {
                 }
             }
         }
-        if (n < 7) { n++; continue; }
+        if (n < 7) {
+            n++;
+            if (it >= 2) prefetch(n);
+            continue;
+        }
         // ---- end of a predictor-corrector iteration ----
         const double pc_err = gmax<G>(maxdb6) / gmax<G>(maxak);
         if (!(pc_err < 1e-16) && !(it > 2 && pc_err_last <= pc_err) && it < 12) {
             pc_err_last = pc_err;
             it++;
             n = 1;
+            prefetch(1);
             continue;
         }
         // ---- step-size control (real particle only) ----
@@ -812,8 +842,9 @@ static const void* ias15_lookup(int G, bool mixed, int waves, bool ext) {
     }
 }
 
-size_t ias15_lds_bytes(int G, int nvec, int threads, int waves) {
-    return (kTabWords + ((size_t)(threads / G) * ias15_cols_per_group(G, nvec, waves) + 1) * kLdsColWords) *
+size_t ias15_lds_bytes(int G, int nvec, int threads, int waves, bool ext) {
+    return (kTabWords + ((size_t)(threads / G) * ias15_cols_per_group(G, nvec, waves) + 1) * kLdsColWords +
+            (size_t)(threads / 64) * ias15_xch_words(G, ext)) *
            sizeof(double);
 }
 
@@ -821,7 +852,7 @@ hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int waves, in
                         bool set_attributes) {
     const void* f = ias15_lookup(G, mixed, waves, ext_model(a.fo));
     if (!f) return hipErrorInvalidValue;
-    const size_t lds = ias15_lds_bytes(G, a.nvec, threads, waves);
+    const size_t lds = ias15_lds_bytes(G, a.nvec, threads, waves, ext_model(a.fo));
     hipError_t e = hipSuccess;
     if (set_attributes) {  // (set beforehand by ias15_occupancy when the launch is being captured)
         e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -836,7 +867,7 @@ hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int waves, in
 hipError_t ias15_occupancy(int G, bool mixed, int waves, bool ext, int nvec, int threads, int* blocks_per_cu) {
     const void* f = ias15_lookup(G, mixed, waves, ext);
     if (!f) return hipErrorInvalidValue;
-    const size_t lds = ias15_lds_bytes(G, nvec, threads, waves);
+    const size_t lds = ias15_lds_bytes(G, nvec, threads, waves, ext);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, threads, lds);

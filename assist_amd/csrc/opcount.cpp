@@ -9,8 +9,8 @@
 // fabs count 0.  The IAS15 arithmetic is replayed from the kernel's expressions.
 //
 // Output (JSON): FP64 and FP32 operations per force evaluation for the force
-// configurations used by the benchmarks, and the IAS15 operations per
-// component and per system.
+// configurations used by the benchmarks, the Chebyshev share of an evaluation,
+// and the IAS15 operations per component and per system.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -113,6 +113,17 @@ static Counts count_eval(const DevEphem& E, int forces, int nvec, bool ng) {
         own_accel<Cnt, TS>(v, R, J, xr, vr, dA, ng, out);
     }
     return Counts{g_ops64, g_ops32};
+}
+
+// Chebyshev operations of one force evaluation (the 28 slots' positions, and the Sun's velocity
+// when a term needs it): the part an evaluation served by the substep ephemeris cache skips.
+static long long count_cheb(const DevEphem& E, bool sun_vel) {
+    g_ops64 = g_ops32 = 0;
+    for (int s = 0; s < kNumSlots; s++) {
+        if (s == 0 && sun_vel) (void)cheb_eval<Cnt, true>(E.slot[s], Cnt(1.0));
+        else (void)cheb_eval<Cnt, false>(E.slot[s], Cnt(1.0));
+    }
+    return g_ops64;
 }
 
 // IAS15 arithmetic of the kernel (kernels.hip), replayed with the counting type.
@@ -236,6 +247,7 @@ int main() {
         }
     }
     printf("\n  },\n");
+    printf("  \"cheb\": {\"sun_vel\": %lld, \"no_sun_vel\": %lld},\n", count_cheb(E, true), count_cheb(E, false));
     const IasCounts c = count_ias15();
     long long corr = 0;
     for (int n = 1; n <= 7; n++) corr += c.correct_comp[n];

@@ -254,6 +254,9 @@ struct assist_batch_config {
     int use_graph;        /* 1: launch the integration as a captured HIP graph, captured once per
                              argument set and replayed (the launches of a replay take the same
                              device pointers); 0 (default): direct launches */
+    int substep_cache_off; /* 0 (default): 8-lane groups keep the ephemeris positions of each
+                             substep for the later predictor-corrector iterations of the step
+                             (results are the same bit for bit); 1: evaluate them every time */
 };
 
 struct assist_batch_stats {

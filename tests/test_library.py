@@ -195,7 +195,8 @@ def test_batch_config_default_sets_every_field():
     C.memset(C.addressof(c), 0x5A, C.sizeof(c))
     assist_amd.clibassist.assist_batch_config_default(C.byref(c))
     assert (c.epsilon, c.min_dt, c.dt_initial, c.max_steps, c.precision) == (1e-9, 0.0, 0.001, 1000000, 0)
-    assert (c.lanes_per_system, c.block_size, c.blocks_per_cu, c.waves_per_simd, c.use_graph) == (0, 0, 0, 0, 0)
+    assert (c.lanes_per_system, c.block_size, c.blocks_per_cu, c.waves_per_simd, c.use_graph,
+            c.substep_cache_off) == (0, 0, 0, 0, 0, 0)
 
 
 def test_gpu_entry_points_fail_without_gpu(ephem):
