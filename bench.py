@@ -103,36 +103,77 @@ def make_workload(cfg, model):
     return t0, t1, states, params
 
 
-def cpu_baseline(paths, cfg, t0, t1, states, params, target_cpu_s=20.0):
-    """The C oracle on the host cores over a bounded sample of the same workload."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_lib import Oracle
-    o = Oracle(*paths)
-    cores = min(os.cpu_count() or 1, 16)
-    probe = 16
-    tp = time.perf_counter()
-    o.integrate(t0[:probe], t1[:probe], states[:probe], None if params is None else params[:probe], nthreads=1,
-                cfg=o.cfg(min_dt=cfg["min_dt"]))
-    per_sys = (time.perf_counter() - tp) / probe
-    m = int(min(len(t0), max(cores, target_cpu_s / max(per_sys, 1e-6))))
-    sel = slice(0, m)
-    ts = time.perf_counter()
-    r = o.integrate(t0[sel], t1[sel], states[sel], None if params is None else params[sel], nthreads=cores,
-                    cfg=o.cfg(min_dt=cfg["min_dt"]))
-    el = time.perf_counter() - ts
-    pdays = float(np.sum(np.abs(t1[sel] - t0[sel])))
-    cpu_model = ""
+def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return dict(value=pdays / el, unit="particle-days/s", cores=cores, kind="port",
-                sample="first %d systems of the workload (%.0f particle-days) on %d threads, %s" % (m, pdays, cores, cpu_model),
-                seconds=el), r, m
+    return ""
+
+
+# Initial state of the reference's benchmark particle (reference unit_tests/benchmark/problem.c:72-76);
+# particle i is offset by i / 1e-10 au in x, as there.
+REFERENCE_BENCHMARK_STATE = [-2.724183384883979E+00, -3.523994546329214E-02, 9.036596202793466E-02,
+                             -1.374545432301129E-04, -1.027075301472321E-02, -4.195690627695180E-03]
+
+
+def cpu_reference_shape(o, samples=5):
+    """The reference's own benchmark shape on the C oracle: 10 particles in one simulation sharing
+    one IAS15 step, 10 years, default forces, one thread, mean and standard deviation of 5 samples
+    (reference unit_tests/benchmark/problem.c:60-95, 130-149).  It starts at t = 0 (JD 2451545.0)
+    rather than t = 8416.5, which with 3652.5 days lies beyond the synthetic ephemeris' +-10 000 d."""
+    reals = np.tile(np.array(REFERENCE_BENCHMARK_STATE), (10, 1))
+    reals[:, 0] += np.arange(10) / 1e-10
+    secs = []
+    for _ in range(samples):
+        ts = time.perf_counter()
+        _, st = o.integrate_simulation(0.0, 3652.5, reals)
+        secs.append(time.perf_counter() - ts)
+    mean, std = float(np.mean(secs)), float(np.std(secs, ddof=1))
+    return dict(seconds_mean=mean, seconds_std=std, samples=samples, particle_days_per_s=10 * 3652.5 / mean,
+                steps=int(st.steps), evals=int(st.evals),
+                note="synthetic ephemeris, t0 = 0; the reference reports 0.182 s on DE440 on its authors' laptop")
+
+
+def cpu_baseline(paths, cfg, t0, t1, states, params, sample_s=3.0, samples=5):
+    """The C oracle on the host cores over a bounded sample of the same workload: `samples` timed
+    runs of the first m systems on all threads (mean and standard deviation, the reference's
+    protocol), one run on one thread, and the reference's benchmark shape."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    o = Oracle(*paths)
+    cores = min(os.cpu_count() or 1, 16)
+    prm = (lambda sl: None) if params is None else (lambda sl: params[sl])
+    ocfg = o.cfg(min_dt=cfg["min_dt"])
+    probe = 16
+    tp = time.perf_counter()
+    o.integrate(t0[:probe], t1[:probe], states[:probe], prm(slice(0, probe)), nthreads=1, cfg=ocfg)
+    per_sys = (time.perf_counter() - tp) / probe
+    m = int(min(len(t0), max(cores, cores * sample_s / max(per_sys, 1e-6))))
+    sel = slice(0, m)
+    pdays = float(np.sum(np.abs(t1[sel] - t0[sel])))
+    rates = []
+    for k in range(samples):
+        ts = time.perf_counter()
+        rk = o.integrate(t0[sel], t1[sel], states[sel], prm(sel), nthreads=cores, cfg=ocfg)
+        rates.append(pdays / (time.perf_counter() - ts))
+        if k == 0:
+            r = rk
+    m1 = int(min(len(t0), max(1, sample_s / max(per_sys, 1e-6))))
+    s1 = slice(0, m1)
+    ts = time.perf_counter()
+    o.integrate(t0[s1], t1[s1], states[s1], prm(s1), nthreads=1, cfg=ocfg)
+    rate1 = float(np.sum(np.abs(t1[s1] - t0[s1]))) / (time.perf_counter() - ts)
+    cpu = _cpu_model()
+    return dict(value=float(np.mean(rates)), std=float(np.std(rates, ddof=1)), samples=samples,
+                unit="particle-days/s", cores=cores, kind="port",
+                sample="first %d systems of the workload (%.0f particle-days) on %d threads, %d runs, %s"
+                       % (m, pdays, cores, samples, cpu),
+                single_thread={"value": rate1, "sample": "first %d systems on 1 thread" % m1},
+                reference_benchmark_shape=cpu_reference_shape(o)), r, m
 
 
 def copy_calibration(mib):
@@ -310,7 +351,6 @@ def main():
     if not args.no_cpu_baseline:
         cb, r, m = cpu_baseline(paths, cfg, t0, t1, states, params)
         dcpu = np.linalg.norm(plain["states"][:m, 0, :3] - r["states"][:, 0, :3], axis=-1) * 149597870700.0
-        cb.pop("seconds", None)
         result["cpu_baseline"] = cb
         result["parity_vs_oracle_sample"] = {"systems": m, "form": "fp64", "max_position_diff_m": float(dcpu.max())}
     print(json.dumps(result), flush=True)

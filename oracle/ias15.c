@@ -193,6 +193,7 @@ int orc_integrate(const orc_ephem *eph, const orc_opts *o, orc_ias15_cfg *cfg, o
         }
     double t = s->t;
     const double dir = (t_end >= t) ? 1.0 : -1.0;
+    orc_cache_direction(dir > 0 ? 1 : -1);
     double dt = cfg->dt;
     if (dt * dir < 0) dt = -dt;
     double dt_last_success = 0.0;

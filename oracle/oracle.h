@@ -102,6 +102,8 @@ void orc_ephem_free(orc_ephem *e);
 int orc_ephem_num_asteroids(const orc_ephem *e);
 /* Body i at t (days rel. jd_ref): out = GM, x,y,z, vx,vy,vz, ax,ay,az (au, au/d, au/d^2). */
 int orc_all_ephem(const orc_ephem *e, int i, double t, double out[10]);
+/* Direction of integration for the ephemeris cache's replacement rule (this thread). */
+void orc_cache_direction(int dir);
 void orc_time_bounds(const orc_ephem *e, double *t_beg, double *t_end);
 int orc_spk_eval_target(const orc_ephem *e, int which_file, int target_idx, double t, double out[9]);
 

@@ -10,6 +10,9 @@
 
 #include "oracle.h"
 
+/* Bumped on every ephemeris load or free: empties the per-thread ephemeris caches (below). */
+static volatile long orc_generation = 1;
+
 #define REC 1024
 
 /* reference src/spk.c:133-134 */
@@ -219,6 +222,7 @@ static orc_ephem *ascii_ephem_load(orc_ephem *e, const char *planets_path, const
 }
 
 orc_ephem *orc_ephem_load(const char *planets_path, const char *asteroids_path, int *err) {
+    orc_generation++;
     orc_ephem *e = calloc(1, sizeof(orc_ephem));
     e->jd_ref = 2451545.0;
     *err = ORC_SUCCESS;
@@ -253,6 +257,7 @@ orc_ephem *orc_ephem_load(const char *planets_path, const char *asteroids_path, 
 
 void orc_ephem_free(orc_ephem *e) {
     if (!e) return;
+    orc_generation++;
     spk_free(e->planets);
     spk_free(e->asteroids);
     orc_ascii_free(e->ascii);
@@ -392,7 +397,7 @@ static int planet_calc(const orc_ephem *e, double jd_ref, double t, int body, do
 }
 
 /* reference src/forces.c:175-263 (without the transparent 7-slot cache) */
-int orc_all_ephem(const orc_ephem *e, int i, double t, double out[10]) {
+static int all_ephem_eval(const orc_ephem *e, int i, double t, double out[10]) {
     const double jd_ref = e->jd_ref;
     if (i < ORC_NPLANETS) {
         int f = planet_calc(e, jd_ref, t, i, &out[0], out + 1);
@@ -409,6 +414,52 @@ int orc_all_ephem(const orc_ephem *e, int i, double t, double out[10]) {
         out[3] = x[2] + s[3];
         for (int k = 4; k < 10; k++) out[k] = NAN;
     }
+    return ORC_SUCCESS;
+}
+
+/* Ephemeris cache, as reference src/forces.c:175-263: up to 7 results per body keyed by the
+ * exact time, which IAS15's repeated substep times hit; a miss fills an empty entry or replaces
+ * the one whose time is oldest in the direction of integration.  An entry holds exactly what the
+ * evaluation returned, so results are unchanged.  The cache is per thread and is emptied when
+ * any ephemeris is loaded or freed. */
+#define ORC_CACHE_BODIES 64
+typedef struct {
+    long gen;
+    const orc_ephem *e;
+    int used[ORC_CACHE_BODIES];
+    double t[ORC_CACHE_BODIES][7];
+    double v[ORC_CACHE_BODIES][7][10];
+} orc_cache_t;
+static __thread orc_cache_t orc_tc;
+static __thread int orc_tc_dir = 1;
+
+void orc_cache_direction(int dir) { orc_tc_dir = dir >= 0 ? 1 : -1; }
+
+int orc_all_ephem(const orc_ephem *e, int i, double t, double out[10]) {
+    if (i < 0 || i >= ORC_CACHE_BODIES) return all_ephem_eval(e, i, t, out);
+    orc_cache_t *c = &orc_tc;
+    if (c->gen != orc_generation || c->e != e) {
+        c->gen = orc_generation;
+        c->e = e;
+        memset(c->used, 0, sizeof c->used);
+    }
+    for (int k = 0; k < c->used[i]; k++)
+        if (c->t[i][k] == t) {
+            memcpy(out, c->v[i][k], 10 * sizeof(double));
+            return ORC_SUCCESS;
+        }
+    const int f = all_ephem_eval(e, i, t, out);
+    if (f != ORC_SUCCESS) return f;
+    int slot = c->used[i];
+    if (slot < 7) {
+        c->used[i]++;
+    } else {
+        slot = 0;
+        for (int k = 1; k < 7; k++)
+            if ((orc_tc_dir > 0) ? (c->t[i][k] < c->t[i][slot]) : (c->t[i][k] > c->t[i][slot])) slot = k;
+    }
+    c->t[i][slot] = t;
+    memcpy(c->v[i][slot], out, 10 * sizeof(double));
     return ORC_SUCCESS;
 }
 

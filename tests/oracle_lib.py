@@ -34,6 +34,16 @@ class OrcEphem(C.Structure):
                                   "Re_eq", "Rs_eq", "c_AU_per_day", "c_squared", "over_c_squared")]
 
 
+class OrcSys(C.Structure):
+    _fields_ = [("n_real", C.c_int), ("n_var", C.c_int), ("var_tp", C.c_void_p), ("t", C.c_double),
+                ("p", C.c_void_p), ("params", C.c_void_p), ("status", C.c_int)]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [("steps", C.c_long), ("rejected", C.c_long), ("iterations", C.c_long), ("evals", C.c_long),
+                ("pc_maxed", C.c_long), ("status", C.c_int), ("dt_last_done", C.c_double), ("dt_next", C.c_double)]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
@@ -59,6 +69,8 @@ def lib():
         L.orc_batch_forces.argtypes = [C.POINTER(OrcEphem), C.POINTER(OrcOpts), C.c_int, C.c_int, _dp, _dp, _dp, _dp, _ip]
         L.orc_batch_integrate.argtypes = [C.POINTER(OrcEphem), C.POINTER(OrcOpts), C.POINTER(OrcIas15Cfg), C.c_int, C.c_int,
                                           _dp, _dp, _dp, _dp, C.c_int, _ip, _lp, _lp, _lp, _dp]
+        L.orc_integrate.argtypes = [C.POINTER(OrcEphem), C.POINTER(OrcOpts), C.POINTER(OrcIas15Cfg), C.POINTER(OrcSys),
+                                    C.c_double, C.POINTER(OrcStats)]
         _lib = L
     return _lib
 
@@ -149,6 +161,20 @@ class Oracle:
                                   st.ctypes.data_as(_ip), steps.ctypes.data_as(_lp), evals.ctypes.data_as(_lp),
                                   rej.ctypes.data_as(_lp), _d(tout))
         return dict(states=states, status=st, steps=steps, evals=evals, rejected=rej, t=tout)
+
+
+    def integrate_simulation(self, t0, t1, reals, opts=None, cfg=None):
+        """One reference-style simulation: the real particles reals (N, 6) share one IAS15 step
+        (no variational particles), from t0 to t1; returns the final states and the stats."""
+        reals = np.asarray(reals, dtype=np.float64)
+        n = reals.shape[0]
+        p = np.zeros((n, 9))
+        p[:, :6] = reals
+        s = OrcSys(n_real=n, n_var=0, var_tp=None, t=float(t0), p=p.ctypes.data, params=None, status=0)
+        st = OrcStats()
+        c = cfg or self.cfg()
+        lib().orc_integrate(self.e, C.byref(opts or self.opts()), C.byref(c), C.byref(s), float(t1), C.byref(st))
+        return p[:, :6].copy(), st
 
 
 def ias15_tables():

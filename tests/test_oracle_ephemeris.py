@@ -144,3 +144,26 @@ def test_record_joins_are_continuous(ephem_paths, which, idx, tol_rel):
         left = np.array([C.chebval(1.0, r0[2 + c * P: 2 + (c + 1) * P]) for c in range(3)])
         right = np.array([C.chebval(-1.0, r1[2 + c * P: 2 + (c + 1) * P]) for c in range(3)])
         assert np.abs(left - right).max() < tol_rel * np.abs(left).max()
+
+
+def test_ephemeris_cache_is_transparent(ephem_paths):
+    """The oracle's per-thread ephemeris cache (reference src/forces.c:175-263: 7 entries per body
+    keyed by the exact time) returns exactly what a fresh evaluation returns: for a body queried
+    again after 20 other times (its entries replaced), at a time still cached, and after the
+    ephemeris is loaded again (which empties the cache)."""
+    from oracle_lib import Oracle
+    o = Oracle(*ephem_paths)
+    rng = np.random.default_rng(5)
+    times = rng.uniform(-9000.0, 9000.0, 20)
+    for body in (0, 3, 10, 11, 26):
+        f0, first = o.all_ephem(body, 123.456)
+        assert f0 == 0
+        for t in times:
+            o.all_ephem(body, t)
+        _, again = o.all_ephem(body, 123.456)          # evaluated afresh (entry replaced)
+        _, cached = o.all_ephem(body, times[-1])        # served from the cache
+        fresh = Oracle(*ephem_paths)                    # a new load empties the cache
+        _, ref = fresh.all_ephem(body, times[-1])
+        assert np.array_equal(first, again, equal_nan=True)
+        assert np.array_equal(cached, ref, equal_nan=True)
+        del fresh
