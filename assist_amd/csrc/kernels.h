@@ -72,7 +72,9 @@ struct IntegrateArgs {
 };
 
 // Doubles of the substep ephemeris cache per launched thread (0 when the kernel keeps none):
-// 8-lane groups outside the extended model keep kEcWords per substep and group.
+// 8-lane groups outside the extended model keep 88 words per substep and group.  (One lane per
+// system keeps none: 64 systems per wave need their slots evaluated on nearly every trip, and
+// the lanes that would load instead wait on memory for all 28 slots.)
 __host__ __device__ constexpr int ias15_cache_words(int G, bool ext) {
     return (G == 8 && !ext) ? 7 * 88 / 8 : 0;
 }

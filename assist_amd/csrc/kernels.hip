@@ -183,7 +183,7 @@ ias15_kernel(IntegrateArgs args) {
     bool active = grp < ka->nsys;
     int sys = active ? (ka->order ? ka->order[grp] : grp) : 0;
     int cur = 0, cend = 0;  // dense output: next and end index of this system's output times
-    constexpr bool kWaveRounds = ias15_cache_words(G, Ext) > 0;
+    constexpr bool kWaveRounds = ias15_cache_words(G, Ext) > 0;  // 8-lane groups
     SubstepCache scache;
     // the next trip is a later iteration at substep nn: load its cached positions now
     auto prefetch = [&](int nn) {

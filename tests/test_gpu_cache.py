@@ -17,62 +17,72 @@ from conftest import population
 pytestmark = pytest.mark.gpu
 
 
-def _pair(ephem, t0, t1, states, params, **cfg):
+# lanes per system and variational particles per system of each case: 8-lane groups with 6
+# variational particles, and with the real particle only (7 lanes of each group own no vector)
+SHAPES = [(8, 6), (8, 0)]
+
+
+def _pair(ephem, t0, t1, states, params, lanes, **cfg):
     from assist_amd import batch
-    on = batch.integrate(ephem, t0, t1, states, params, config=batch.BatchConfig.default(lanes_per_system=8, **cfg))
+    on = batch.integrate(ephem, t0, t1, states, params, config=batch.BatchConfig.default(lanes_per_system=lanes, **cfg))
     off = batch.integrate(ephem, t0, t1, states, params,
-                          config=batch.BatchConfig.default(lanes_per_system=8, substep_cache_off=1, **cfg))
+                          config=batch.BatchConfig.default(lanes_per_system=lanes, substep_cache_off=1, **cfg))
     return on, off
 
 
-def _same(on, off):
+def _same(on, off, lanes):
     assert (on.status == 0).all() and (off.status == 0).all()
     assert np.array_equal(on.states, off.states)
     assert np.array_equal(on.steps, off.steps)
     assert (on.stats.evals, on.stats.rejected) == (off.stats.evals, off.stats.rejected)
-    assert on.stats.lanes_per_system == 8
+    assert on.stats.lanes_per_system == lanes
 
 
+@pytest.mark.parametrize("lanes,n_var", SHAPES)
 @pytest.mark.parametrize("precision", [0, 1])
-def test_cache_equal_spans(gpu, ephem, model, precision):
-    states, _, _ = population(model, 3000, 41)
+def test_cache_equal_spans(gpu, ephem, model, precision, lanes, n_var):
+    states, _, _ = population(model, 3000, 41, n_var=n_var)
     n = states.shape[0]
-    on, off = _pair(ephem, np.zeros(n), np.full(n, 120.0), states, None, precision=precision, min_dt=0.001)
-    print("equal spans, precision %d: %d steps, %d evaluations, %d rejected"
-          % (precision, on.stats.steps, on.stats.evals, on.stats.rejected))
-    _same(on, off)
+    on, off = _pair(ephem, np.zeros(n), np.full(n, 120.0), states, None, lanes, precision=precision, min_dt=0.001)
+    print("equal spans, %d lanes, precision %d: %d steps, %d evaluations, %d rejected"
+          % (lanes, precision, on.stats.steps, on.stats.evals, on.stats.rejected))
+    _same(on, off, lanes)
 
 
-def test_cache_uneven_spans_marsden(gpu, ephem, model):
+@pytest.mark.parametrize("lanes,n_var", SHAPES)
+def test_cache_uneven_spans_marsden(gpu, ephem, model, lanes, n_var):
     # ragged tail: groups leave their wave at different times (partly active waves), half of
     # the systems integrate backward, 30% carry a Marsden model
     rng = np.random.default_rng(42)
-    states, params, _ = population(model, 2500, 42, ng_fraction=0.3)
+    states, params, _ = population(model, 2500, 42, n_var=n_var, ng_fraction=0.3)
     n = states.shape[0]
     span = np.exp(rng.uniform(np.log(5.0), np.log(400.0), n)) * np.where(rng.random(n) < 0.5, -1.0, 1.0)
     t0 = rng.uniform(-200.0, 200.0, n)
-    on, off = _pair(ephem, t0, t0 + span, states, params, min_dt=0.001)
-    print("uneven spans: %d steps, %d evaluations, %d rejected" % (on.stats.steps, on.stats.evals, on.stats.rejected))
-    _same(on, off)
+    on, off = _pair(ephem, t0, t0 + span, states, params, lanes, min_dt=0.001)
+    print("uneven spans, %d lanes: %d steps, %d evaluations, %d rejected"
+          % (lanes, on.stats.steps, on.stats.evals, on.stats.rejected))
+    _same(on, off, lanes)
 
 
-def test_cache_few_systems(gpu, ephem, model):
+@pytest.mark.parametrize("lanes,n_var", SHAPES)
+def test_cache_few_systems(gpu, ephem, model, lanes, n_var):
     # fewer systems than groups in one wave, so most lanes of the grid never start
-    states, _, _ = population(model, 3, 43)
-    on, off = _pair(ephem, np.zeros(3), np.array([30.0, -45.0, 10.0]), states, None)
-    _same(on, off)
+    states, _, _ = population(model, 3, 43, n_var=n_var)
+    on, off = _pair(ephem, np.zeros(3), np.array([30.0, -45.0, 10.0]), states, None, lanes)
+    _same(on, off, lanes)
 
 
-def test_cache_dense_output(gpu, ephem, model):
+@pytest.mark.parametrize("lanes,n_var", SHAPES)
+def test_cache_dense_output(gpu, ephem, model, lanes, n_var):
     from assist_amd import batch
-    states, _, _ = population(model, 600, 44)
+    states, _, _ = population(model, 600, 44, n_var=n_var)
     n = states.shape[0]
     rng = np.random.default_rng(44)
     outs = [np.sort(rng.uniform(0.0, 90.0, 5)) for _ in range(n)]
     on = batch.integrate_dense(ephem, np.zeros(n), outs, states,
-                               config=batch.BatchConfig.default(lanes_per_system=8))
+                               config=batch.BatchConfig.default(lanes_per_system=lanes))
     off = batch.integrate_dense(ephem, np.zeros(n), outs, states,
-                                config=batch.BatchConfig.default(lanes_per_system=8, substep_cache_off=1))
+                                config=batch.BatchConfig.default(lanes_per_system=lanes, substep_cache_off=1))
     assert (on.status == 0).all()
     assert np.array_equal(on.states, off.states)
     for a, b in zip(on.outputs, off.outputs):
