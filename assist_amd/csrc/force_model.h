@@ -327,12 +327,12 @@ __device__ __forceinline__ void ec_prefetch(SubstepCache& sc, int lane, int n) {
         const int sl = s < kNumSlots ? s : 0;  // (lanes past the last slot read slot 0)
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const double v = __builtin_nontemporal_load(p + 3 * sl + c);
+            const double v = p[3 * sl + c];
             sc.px[r][c] = (s < kNumSlots) ? v : 0.0;
         }
     }
 #pragma unroll
-    for (int c = 0; c < 3; c++) sc.sv[c] = __builtin_nontemporal_load(p + 84 + c);
+    for (int c = 0; c < 3; c++) sc.sv[c] = p[84 + c];
 }
 
 // Raw positions (km) of the slots r * 8 + lane (r < 4) of this lane's group and, on the group's
@@ -379,14 +379,8 @@ __device__ __forceinline__ void wave_positions(const DevSlot* slots, int lane, d
             }
             if ((nm & 3) == kEcFill) {
                 double* p = sc.base + ((size_t)gi * 7 + ((nm >> 2) - 1)) * kEcWords;
-                __builtin_nontemporal_store(c.x, p + 3 * s);
-                __builtin_nontemporal_store(c.y, p + 3 * s + 1);
-                __builtin_nontemporal_store(c.z, p + 3 * s + 2);
-                if (s == 0) {
-                    __builtin_nontemporal_store(c.vx, p + 84);
-                    __builtin_nontemporal_store(c.vy, p + 85);
-                    __builtin_nontemporal_store(c.vz, p + 86);
-                }
+                p[3 * s] = c.x; p[3 * s + 1] = c.y; p[3 * s + 2] = c.z;
+                if (s == 0) { p[84] = c.vx; p[85] = c.vy; p[86] = c.vz; }
             }
         }
     }
