@@ -13,6 +13,7 @@ import ctypes as C
 import os
 import re
 import struct
+import sys
 import tempfile
 
 import numpy as np
@@ -268,3 +269,21 @@ def test_batched_queries_fail_without_gpu(ephem):
         pytest.skip("a GPU is visible")
     with pytest.raises(batch.BatchError):
         batch.get_particles(ephem, [0], [0.0])
+
+
+def test_operation_count_of_the_chebyshev_series():
+    """The bench's algorithmic count leaves out the Chebyshev series of the evaluations that the
+    substep cache serves; pin that share against a hand count of the evaluation formula.  A
+    position-only evaluation of P coefficients costs 2 (record index) + 2 (z) + 1 (2z) + 3 x 2
+    (first fused term) + (P - 2) x (1 fma for T + 3 fma) x 2 = 11 + 8 (P - 2) operations; the
+    velocity adds (P - 2) x (2 fma for S + 3 fma) x 2 = 10 (P - 2) and 3 scalings.  The benchmark
+    ephemeris has 10 slots of 14 coefficients, Earth and Moon of 13 and 16 asteroids of 11."""
+    sys.path.insert(0, ROOT)
+    import bench
+    oc = bench.opcounts()
+    shapes = [14] * 10 + [13] * 2 + [11] * 16
+    no_vel = sum(11 + 8 * (p - 2) for p in shapes)
+    assert oc["cheb"]["no_sun_vel"] == no_vel == 2596
+    assert oc["cheb"]["sun_vel"] == no_vel + 10 * (14 - 2) + 3
+    # the series are part of the full evaluation count they are subtracted from
+    assert oc["cheb"]["sun_vel"] < oc["eval"]["nvec7_ng0_fp64"]["fp64"]
