@@ -176,7 +176,9 @@ ias15_kernel(IntegrateArgs args) {
                           (size_t)(threadIdx.x >> 6) * ias15_xch_words(G, Ext);
     double A[3], dA[3];
     double t = 0, t_end = 0, dt = 0, dts = 0, dt_last = 0, pc_err_last = 0;
-    int phase = 0, n = 1, it = 0, last = 0, status = 0, attempts = 0;
+    // (phase: false at a step's first evaluation, true in its predictor-corrector substeps)
+    bool phase = false;
+    int n = 1, it = 0, last = 0, status = 0, attempts = 0;
     long long steps = 0, evals = 0, rejected = 0;
     bool ng = false;
     // the queue hands out positions q; order[q] (when given) is the system at that position
@@ -226,7 +228,7 @@ ias15_kernel(IntegrateArgs args) {
         dt = ka->dt0 ? fabs(ka->dt0[s]) : ka->io.dt_initial;
         if (dt * (t_end - t) < 0) dt = -dt;
         dt_last = 0.0;
-        phase = 0;
+        phase = false;
         status = 0;
         steps = evals = rejected = 0;
         last = 0;
@@ -313,7 +315,7 @@ ias15_kernel(IntegrateArgs args) {
         it = 1;
         pc_err_last = 1e300;
         n = 1;
-        phase = 1;
+        phase = true;
         return true;
     };
 
@@ -322,10 +324,10 @@ ias15_kernel(IntegrateArgs args) {
         ka = launder(ka);
         const DevEphem& E = *ka->E;
         // zero-length span (dense output: no output times left)
-        if (phase == 0 && (ka->out_t ? cur >= cend : t == t_end)) { finish(); continue; }
+        if (!phase && (ka->out_t ? cur >= cend : t == t_end)) { finish(); continue; }
         // ---- evaluation point of this trip ----
         double xe[3], ve[3], te;
-        if (phase == 0) {
+        if (!phase) {
 #pragma unroll
             for (int c = 0; c < 3; c++) { xe[c] = x0[c]; ve[c] = v0[c]; }
             te = t;
@@ -366,7 +368,7 @@ ias15_kernel(IntegrateArgs args) {
             scache.base = ka->ecache ? ka->ecache + (size_t)(tid >> 6) * (64 * ias15_cache_words(G, Ext)) : nullptr;
             scache.xch = s_xch;
             scache.n = n;
-            scache.mode = (!ka->ecache || phase == 0) ? kEcEval : (it == 1 ? kEcFill : kEcUse);
+            scache.mode = (!ka->ecache || !phase) ? kEcEval : (it == 1 ? kEcFill : kEcUse);
             scp = &scache;
         }
         force_eval<G, Mixed, double, Ext>(E, s_slots, s_gm, fo, lane, te, xr, vr, A, ng, nv > 1, R, J, scp);
@@ -375,7 +377,7 @@ ias15_kernel(IntegrateArgs args) {
         evals++;
         if (R.status != kOk) { status = R.status; finish(); continue; }
 
-        if (phase == 0) {
+        if (!phase) {
 #pragma unroll
             for (int c = 0; c < 3; c++) a0[c] = at[c];
             if (!setup_attempt(true)) { finish(); continue; }
@@ -574,7 +576,7 @@ ias15_kernel(IntegrateArgs args) {
         if (bad) { status = kNonFinite; finish(); continue; }
         if (last || (ka->out_t && cur >= cend)) { finish(); continue; }
         if (steps >= ka->io.max_steps) { status = kMaxSteps; finish(); continue; }
-        phase = 0;
+        phase = false;
     }
 }
 
