@@ -419,9 +419,11 @@ struct LaunchShape {
 };
 
 constexpr int kTuneMinSystems = 16384;  // smaller batches take the default shape
-// Systems per trial launch: several resident rounds even at one lane per system (65,536 lanes
-// fill the GPU), so that every shape is timed at throughput rather than by its slowest system.
-constexpr int kTuneSample = 262144;
+// Systems per trial launch: many resident rounds even at one lane per system (65,536 lanes fill
+// the GPU), so that every shape is timed at throughput rather than by its slowest systems; with
+// a learned first step a system covers the trial span in a few steps, and 4 systems per lane
+// made the one-lane kernel look slower than 8-lane groups while it is 1.5x faster in full.
+constexpr int kTuneSample = 1048576;
 constexpr double kTuneSpan = 30.0;      // days integrated per trial system
 
 // Replays (capturing it first when new) the launch set_counter + ias15_kernel for `a` on the
@@ -559,10 +561,13 @@ int tuned_shape(const IntegrateArgs& base, bool mixed, hipStream_t s, LaunchShap
     LaunchShape best;
     best.G = lanes_for(base.nvec, 0);
     float best_ms = 1e30f;
-    for (int G = 1; G <= 8; G <<= 1) {
+    // candidates: one lane per system (real-only systems) and 8-lane groups, 1 wave per SIMD;
+    // 2- and 4-lane groups and the 2-wave variant were slower in every measured configuration
+    // (they stay available through assist_batch_config)
+    for (int G : {1, 8}) {
         if (G < base.nvec) continue;
         for (int threads : {256, 128}) {
-            for (int waves : {1, 2}) {
+            for (int waves : {1}) {
                 LaunchShape sh;
                 sh.G = G;
                 sh.threads = threads;
