@@ -1088,8 +1088,9 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
     };
     if constexpr (kStream) {
         // one system per lane: each slot is evaluated and folded in at once (the Sun and the
-        // EMB, which the others need, were evaluated above)
-#pragma unroll 2
+        // EMB, which the others need, were evaluated above); four slots in flight (3% faster
+        // than two on 1,048,576 real-only systems)
+#pragma unroll 4
         for (int s = 1; s < kNumSlots; s++) {
             if (s == kSlotEMB) continue;
             const ChebOut<D> c = cheb_eval<D, false>(slots[s], t);
