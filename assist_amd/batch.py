@@ -203,6 +203,17 @@ class DeviceArray:
             raise BatchError("device allocation failed")
         self.upload(host)
 
+    @classmethod
+    def empty(cls, shape, dtype=np.float64) -> "DeviceArray":
+        """An uninitialised device buffer (for outputs the library writes in full)."""
+        self = cls.__new__(cls)
+        self.shape, self.dtype = tuple(np.atleast_1d(shape)), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = clibassist.assist_device_malloc(max(self.nbytes, 8))
+        if not self.ptr:
+            raise BatchError("device allocation failed")
+        return self
+
     def upload(self, host: np.ndarray):
         host = np.ascontiguousarray(host, dtype=self.dtype)
         _check(clibassist.assist_memcpy_h2d(self.ptr, host.ctypes.data, self.nbytes))

@@ -226,10 +226,20 @@ def train(model: StepModel, X: np.ndarray, T: np.ndarray, epochs=4, batch=8192, 
     return hist
 
 
-def predict_start(ephem, model: StepModel, n, n_var, d_t0, d_t1, d_states, min_dt=0.0, iterations=3, warm=True):
-    """Device arrays (dt0 [n], bw [n, 21] or None) for assist_batch_integrate_device_ex."""
-    d_dt0 = DeviceArray(np.zeros(n))
-    d_bw = DeviceArray(np.zeros((n, 21))) if warm else None
+def predict_start(ephem, model: StepModel, n, n_var, d_t0, d_t1, d_states, min_dt=0.0, iterations=3, warm=True,
+                  out=None):
+    """Device arrays (dt0 [n], bw [n, 21] or None) for assist_batch_integrate_device_ex.  `out`
+    (dt0, bw) reuses buffers of a previous call (every element is written), so that repeated
+    launches keep the same device pointers, as a captured graph replay needs."""
+    if out is not None:
+        d_dt0, d_bw = out
+        if d_dt0.nbytes < 8 * n or (warm and (d_bw is None or d_bw.nbytes < 8 * 21 * n)):
+            raise ValueError("predict_start: the output buffers are smaller than the batch")
+        if not warm:
+            d_bw = None
+    else:
+        d_dt0 = DeviceArray.empty(n)
+        d_bw = DeviceArray.empty((n, 21)) if warm else None
     _check(clibassist.assist_batch_predict_start(C.byref(ephem), C.c_void_p(model.ptr), int(n), int(n_var), _p(d_t0),
                                                  _p(d_t1), _p(d_states), float(min_dt), int(iterations), _p(d_dt0),
                                                  _p(d_bw)))

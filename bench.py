@@ -225,10 +225,15 @@ def run_form(form, args, cfg, ephem, bufs, n, nv, step_model):
                                    use_graph=1 if args.launch == "graph" else 0)
     d_t0, d_t1, d_init, d_states, d_params, d_status, d_steps, d_evals = bufs
 
+    # the learned start's outputs are written in full by every prediction; one pair of buffers
+    # keeps the device pointers of the captured launch the same from step to step
+    start_bufs = (batch.DeviceArray.empty(n), batch.DeviceArray.empty((n, 21))) if spec["learned"] else None
+
     def one_step():
         d_states.copy_from(d_init)
         if spec["learned"]:
-            d_dt0, d_bw = learned.predict_start(ephem, step_model, n, nv - 1, d_t0, d_t1, d_init, min_dt=cfg["min_dt"])
+            d_dt0, d_bw = learned.predict_start(ephem, step_model, n, nv - 1, d_t0, d_t1, d_init, min_dt=cfg["min_dt"],
+                                                out=start_bufs)
             return learned.integrate_device_learned(ephem, n, nv - 1, d_t0, d_t1, d_states, d_params, d_status, d_steps,
                                                     d_evals, d_dt0, d_bw, config=bc)
         return batch.integrate_device(ephem, n, nv - 1, d_t0, d_t1, d_states, d_params, d_status, d_steps, d_evals,
