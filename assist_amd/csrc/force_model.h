@@ -285,8 +285,8 @@ AHD ChebOut<D> cheb_eval(const DevSlot& S, D t) {
 // where some groups of a wave need fresh positions (kEcEval: the step's first evaluation, or
 // kEcFill), their 28 evaluations each are spread slot-major over all active lanes of the wave, so
 // the wave runs ceil(28 k / A) rounds for k such groups and A active lanes instead of 4, and the
-// results reach the owning lanes through ds_bpermute.  Every value comes from the same code and
-// inputs as in the per-group evaluation, so results are unchanged bit for bit.
+// results reach the owning lanes through the wave's LDS exchange area.  Every value comes from the
+// same code and inputs as in the per-group evaluation, so results are unchanged bit for bit.
 enum : int { kEcEval = 0, kEcFill = 1, kEcUse = 2 };
 constexpr int kEcWords = 88;  // per group and substep: raw positions (km) of the 28 slots, then
                               // the Sun's raw velocity (words 84-86), one pad word
