@@ -596,6 +596,12 @@ int tuned_shape(const IntegrateArgs& base, bool mixed, hipStream_t s, LaunchShap
     return ASSIST_SUCCESS;
 }
 
+static int integrate_device_impl(const struct assist_ephem* ephem, const struct assist_extras* opts,
+                                 const struct assist_batch_config* cfg_in, int n, int n_var, const double* d_t0,
+                                 const double* d_t1, double* d_states, const double* d_params, int* d_status,
+                                 long long* d_steps, long long* d_evals, long long* d_rejected, void* stream,
+                                 struct assist_batch_stats* stats, const struct assist_batch_extra* ex);
+
 extern "C" {
 
 void assist_batch_config_default(struct assist_batch_config* cfg) {
@@ -703,6 +709,18 @@ int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const str
                                      const double* d_t1, double* d_states, const double* d_params, int* d_status,
                                      long long* d_steps, long long* d_evals, void* stream,
                                      struct assist_batch_stats* stats, const struct assist_batch_extra* ex) {
+    return integrate_device_impl(ephem, opts, cfg_in, n, n_var, d_t0, d_t1, d_states, d_params, d_status, d_steps,
+                                 d_evals, nullptr, stream, stats, ex);
+}
+
+}  // extern "C"
+
+// The device entry point with a per-system rejection count (d_rejected, or nullptr).
+static int integrate_device_impl(const struct assist_ephem* ephem, const struct assist_extras* opts,
+                                 const struct assist_batch_config* cfg_in, int n, int n_var, const double* d_t0,
+                                 const double* d_t1, double* d_states, const double* d_params, int* d_status,
+                                 long long* d_steps, long long* d_evals, long long* d_rejected, void* stream,
+                                 struct assist_batch_stats* stats, const struct assist_batch_extra* ex) {
     const bool dense = ex && ex->out_t;
     if (n < 0 || n_var < 0 || n_var > 7 || !d_t0 || (!d_t1 && !dense) || !d_states || !d_status)
         return ASSIST_BATCH_ERROR_ARG;
@@ -734,6 +752,7 @@ int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const str
     a.status = d_status;
     a.steps = d_steps;
     a.evals = d_evals;
+    a.rejected = d_rejected;
     if (ex) {
         a.dt0 = ex->dt0;
         a.bw = ex->bw;
@@ -789,6 +808,8 @@ int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const str
     return ASSIST_SUCCESS;
 }
 
+extern "C" {
+
 int assist_batch_get_particles_device(const struct assist_ephem* ephem, int n, const int* d_body, const double* d_t,
                                       double* d_out, int* d_status, void* stream) {
     if (n < 0 || (n > 0 && (!d_body || !d_t || !d_out))) return ASSIST_BATCH_ERROR_ARG;
@@ -836,7 +857,7 @@ int assist_batch_integrate(const struct assist_ephem* ephem, const struct assist
     DevBuf<int> dst;
     DevBuf<long long> dsteps, devals, drej;
     if (!d0.alloc(ns) || !d1.alloc(ns) || !ds.alloc(ns * nv * 6) || !dst.alloc(ns) || !dsteps.alloc(ns) ||
-        !devals.alloc(ns) || (params && !dp.alloc(ns * nv * 3)))
+        !devals.alloc(ns) || !drej.alloc(ns) || (params && !dp.alloc(ns * nv * 3)))
         return ASSIST_BATCH_ERROR_DEVICE;
     if (!hip_ok(hipMemcpy(d0.p, t0, ns * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy") ||
         !hip_ok(hipMemcpy(d1.p, t1, ns * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy") ||
@@ -845,17 +866,18 @@ int assist_batch_integrate(const struct assist_ephem* ephem, const struct assist
         return ASSIST_BATCH_ERROR_DEVICE;
     struct assist_batch_stats st;
     memset(&st, 0, sizeof st);
-    int rc = assist_batch_integrate_device(ephem, opts, cfg, n, n_var, d0.p, d1.p, ds.p, params ? dp.p : nullptr,
-                                           dst.p, dsteps.p, devals.p, nullptr, &st);
+    int rc = integrate_device_impl(ephem, opts, cfg, n, n_var, d0.p, d1.p, ds.p, params ? dp.p : nullptr, dst.p,
+                                   dsteps.p, devals.p, drej.p, nullptr, &st, nullptr);
     if (rc) return rc;
-    std::vector<long long> hsteps(ns), hevals(ns);
+    std::vector<long long> hsteps(ns), hevals(ns), hrej(ns);
     if (!hip_ok(hipMemcpy(states, ds.p, ns * nv * 6 * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy") ||
         !hip_ok(hipMemcpy(hsteps.data(), dsteps.p, ns * sizeof(long long), hipMemcpyDeviceToHost), "hipMemcpy") ||
         !hip_ok(hipMemcpy(hevals.data(), devals.p, ns * sizeof(long long), hipMemcpyDeviceToHost), "hipMemcpy") ||
+        !hip_ok(hipMemcpy(hrej.data(), drej.p, ns * sizeof(long long), hipMemcpyDeviceToHost), "hipMemcpy") ||
         (status && !hip_ok(hipMemcpy(status, dst.p, ns * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy")))
         return ASSIST_BATCH_ERROR_DEVICE;
     if (steps) memcpy(steps, hsteps.data(), ns * sizeof(long long));
-    for (size_t i = 0; i < ns; i++) { st.steps += hsteps[i]; st.evals += hevals[i]; }
+    for (size_t i = 0; i < ns; i++) { st.steps += hsteps[i]; st.evals += hevals[i]; st.rejected += hrej[i]; }
     if (stats) *stats = st;
     return ASSIST_SUCCESS;
 }

@@ -260,7 +260,9 @@ struct assist_batch_config {
 };
 
 struct assist_batch_stats {
-    long long steps, rejected, evals;
+    long long steps, rejected, evals;  /* totals over the batch, filled by assist_batch_integrate;
+                                          the entry points on device buffers leave them 0 (their
+                                          per-system counts are in d_steps / d_evals) */
     double kernel_ms;     /* device time of the integration kernel (HIP events) */
     int lanes_per_system, block_size, grid_blocks, waves_per_simd;
 };

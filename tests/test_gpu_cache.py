@@ -87,3 +87,17 @@ def test_cache_dense_output(gpu, ephem, model, lanes, n_var):
     assert np.array_equal(on.states, off.states)
     for a, b in zip(on.outputs, off.outputs):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("lanes,n_var", SHAPES)
+def test_cache_rejected_steps(gpu, ephem, model, lanes, n_var):
+    # a 500-day first trial step (a third of a main-belt orbit) is rejected and retried with a
+    # smaller one, so every system starts a second attempt of its first step, which must refill
+    # the cache of its substeps
+    states, _, _ = population(model, 1500, 45, n_var=n_var, neo_fraction=0.0)
+    n = states.shape[0]
+    on, off = _pair(ephem, np.zeros(n), np.full(n, 600.0), states, None, lanes, dt_initial=500.0)
+    print("rejections, %d lanes: %d steps, %d evaluations, %d rejected"
+          % (lanes, on.stats.steps, on.stats.evals, on.stats.rejected))
+    assert on.stats.rejected >= n
+    _same(on, off, lanes)
