@@ -315,6 +315,8 @@ def main():
     value = pdays * args.steps / main_res["elapsed"]
     plain_value = pdays * args.steps / plain["elapsed"]
     st = main_res["stats"]
+    # (the device entry points report no rejection total; a rejected attempt's 7 fresh evaluations
+    # then count as cached ones, so the count is a lower bound; these workloads reject rarely)
     f64, f32 = algorithmic_flops(oc, nv, ng, main_res["mixed"], main_res["evals"], main_res["steps"], st.rejected,
                                  cached=st.lanes_per_system == 8)
     kms = main_res["kernel_ms"]
