@@ -287,3 +287,23 @@ def test_operation_count_of_the_chebyshev_series():
     assert oc["cheb"]["sun_vel"] == no_vel + 10 * (14 - 2) + 3
     # the series are part of the full evaluation count they are subtracted from
     assert oc["cheb"]["sun_vel"] < oc["eval"]["nvec7_ng0_fp64"]["fp64"]
+
+
+def test_cpu_baseline_protocol(ephem_paths):
+    """bench.py's CPU baseline on a tiny sample: several timed runs on all threads (mean and
+    standard deviation), one run on one thread, and the reference's benchmark shape (10
+    particles sharing one IAS15 step for 10 years), with the oracle's results returned for the
+    parity check beside the GPU."""
+    sys.path.insert(0, ROOT)
+    import bench
+    from assist_amd import synthetic
+    cfg = dict(bench.CONFIGS["c4"])
+    cfg["n"] = 64
+    t0, t1, states, params = bench.make_workload(cfg, synthetic.SyntheticModel(0))
+    cb, r, m = bench.cpu_baseline(ephem_paths, cfg, t0, t1, states, params, sample_s=0.05, samples=3)
+    assert cb["kind"] == "port" and cb["unit"] == "particle-days/s" and cb["samples"] == 3
+    assert cb["value"] > 0 and cb["std"] >= 0 and cb["cores"] >= 1
+    assert cb["single_thread"]["value"] > 0
+    shape = cb["reference_benchmark_shape"]
+    assert shape["samples"] == 5 and shape["steps"] > 0 and shape["evals"] >= 8 * shape["steps"]
+    assert r["states"].shape == (m, 7, 6) and (r["status"] == 0).all()
