@@ -770,6 +770,15 @@ static int integrate_device_impl(const struct assist_ephem* ephem, const struct 
     shape.G = lanes_for(nv, cfg.lanes_per_system);
     if (shape.G < 0) return ASSIST_BATCH_ERROR_ARG;
     shape.threads = cfg.block_size > 0 ? cfg.block_size : kIas15Block;
+    if (cfg.block_size == 0 && shape.G > 0) {
+        // untuned batches too small to give every CU a 256-thread block take 128-thread blocks,
+        // so that their waves spread over all CUs (configuration 2: 3% faster)
+        int dev = 0, ncu = 0;
+        if (hip_ok(hipGetDevice(&dev), "hipGetDevice") &&
+            hip_ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute") &&
+            (long long)n * shape.G < (long long)kIas15Block * ncu)
+            shape.threads = 128;
+    }
     if (shape.threads % 64 != 0 || shape.threads > kIas15Block || shape.threads % shape.G != 0) return ASSIST_BATCH_ERROR_ARG;
     shape.waves = cfg.waves_per_simd > 0 ? cfg.waves_per_simd : 1;
     if (shape.waves != 1 && shape.waves != 2) return ASSIST_BATCH_ERROR_ARG;
