@@ -271,8 +271,11 @@ int lanes_for(int nvec, int requested) {
         return requested;
     }
     if (g > 8) return -1;
-    if (g < 8 && nvec == 1) g = 8;  // real-only systems: spread the 28 ephemeris targets over 8 lanes
-    return g;
+    // default: 8-lane groups for every vector count (they spread the 28 ephemeris targets over 8
+    // lanes and keep the substep cache; 2- and 4-lane groups, which keep 7-14 slots per lane in
+    // registers, were slower in every measured configuration); the autotuner may pick 1 lane for
+    // large real-only batches
+    return 8;
 }
 
 // Persistent workspace on the current device (grown as needed; one launch at a time).
