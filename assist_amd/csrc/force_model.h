@@ -586,7 +586,12 @@ AHD void term_nongrav(const ForceOpts& O, const D xr[3], const D vr[3], const D 
     const TS alpha = TS(O.alpha), nk = TS(O.nk), nm = TS(O.nm), nn = TS(O.nn), r0 = TS(O.r0);
     const TS r2 = dx * dx + dy * dy + dz * dz;
     const TS r = sqrt(r2);
-    const TS g = alpha * pow(r / r0, -nm) * pow(TS(1.0) + pow(r / r0, nn), -nk);
+    const TS x = r / r0;
+    const TS xm = pow(x, -nm);
+    // nk = 0 (the reference's default, src/assist.c:435): the factor (1 + x^nn)^-nk is pow(., -0),
+    // exactly 1 for every argument, so leaving it out gives g bit for bit.
+    const bool nk0 = nk == TS(0);
+    const TS g = nk0 ? alpha * xm : alpha * xm * pow(TS(1.0) + pow(x, nn), -nk);
     const TS hx = dy * dvz - dz * dvy, hy = dz * dvx - dx * dvz, hz = dx * dvy - dy * dvx;
     const TS h = sqrt(hx * hx + hy * hy + hz * hz);
     const TS tx = hy * dz - hz * dy, ty = hz * dx - hx * dz, tz = hx * dy - hy * dx;
@@ -599,8 +604,20 @@ AHD void term_nongrav(const ForceOpts& O, const D xr[3], const D vr[3], const D 
     const TS v2 = dvx * dvx + dvy * dvy + dvz * dvz;
     const TS rdotv = dx * dvx + dy * dvy + dz * dvz;
     const TS vdott = dvx * tx + dvy * ty + dvz * tz;
-    const TS dgdr = (alpha / r0) * (-nm * pow(r / r0, -nm - TS(1)) * pow(TS(1.0) + pow(r / r0, nn), -nk) +
-                                    pow(r / r0, -nm) * (-nk * nn) * pow(r / r0, nn - TS(1)) * pow(TS(1.0) + pow(r / r0, nn), -nk - TS(1)));
+    // dg/dr = (alpha/r0) (-nm x^(-nm-1) (1+x^nn)^-nk + x^-nm (-nk nn) x^(nn-1) (1+x^nn)^(-nk-1)).
+    // With nk = +0 and nn >= +0 finite, -nk nn = -0; for 0 < x finite the other factors of the
+    // second term are >= +0, so the term is -0 whenever x^-nm and x^(nn-1) are finite, and adding
+    // -0 leaves the first term unchanged bit for bit.  x^(nn-1) is finite when |nn-1| (|e|+1)
+    // stays below the overflow exponent, e the binary exponent of x (x in [2^(e-1), 2^e)); then
+    // the three powers of the second term are skipped.
+    int ex = 0;
+    (void)frexp(x, &ex);
+    const TS e_lim = sizeof(TS) == 8 ? TS(1000) : TS(120);
+    const bool short_dg = nk0 && !signbit(nk) && nn >= TS(0) && !signbit(nn) && isfinite(nn) && x > TS(0) &&
+                          isfinite(x) && isfinite(xm) && fabs(nn - TS(1)) * TS(abs(ex) + 1) < e_lim;
+    const TS dgdr = short_dg ? (alpha / r0) * (-nm * pow(x, -nm - TS(1)))
+                             : (alpha / r0) * (-nm * pow(x, -nm - TS(1)) * pow(TS(1.0) + pow(x, nn), -nk) +
+                                               xm * (-nk * nn) * pow(x, nn - TS(1)) * pow(TS(1.0) + pow(x, nn), -nk - TS(1)));
     const TS dgx = dgdr * dx / r, dgy = dgdr * dy / r, dgz = dgdr * dz / r;
     const TS h3 = h * h * h, t3 = _t * _t * _t;
     const TS hxh3 = hx / h3, hyh3 = hy / h3, hzh3 = hz / h3;

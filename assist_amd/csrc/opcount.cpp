@@ -5,8 +5,9 @@
 // the algorithm needs for one system is counted exactly once (the device
 // replicates the single-particle terms across the lanes of a group; those
 // copies are not algorithmic work).  Counting convention: +, -, *, /, sqrt and
-// pow count 1, a fused multiply-add counts 2, sign changes, comparisons and
-// fabs count 0.  The IAS15 arithmetic is replayed from the kernel's expressions.
+// pow count 1, a fused multiply-add counts 2, sign changes, comparisons, fabs,
+// frexp and class tests count 0.  The IAS15 arithmetic is replayed from the
+// kernel's expressions.
 //
 // Output (JSON): FP64 and FP32 operations per force evaluation for the force
 // configurations used by the benchmarks, the Chebyshev share of an evaluation,
@@ -50,6 +51,17 @@ inline CntF sqrt(CntF a) { g_ops32++; return CntF(::sqrtf(a.v)); }
 inline CntF pow(CntF a, CntF b) { g_ops32++; return CntF(::powf(a.v, b.v)); }
 inline Cnt pow(Cnt a, Cnt b) { g_ops64++; return Cnt(::pow(a.v, b.v)); }
 inline double value_of(Cnt a) { return a.v; }
+#define CMP(T, op) inline bool operator op(T a, T b) { return a.v op b.v; }
+CMP(Cnt, ==) CMP(Cnt, <) CMP(Cnt, >) CMP(Cnt, >=) CMP(CntF, ==) CMP(CntF, <) CMP(CntF, >) CMP(CntF, >=)
+#undef CMP
+inline Cnt fabs(Cnt a) { return Cnt(::fabs(a.v)); }
+inline CntF fabs(CntF a) { return CntF(::fabsf(a.v)); }
+inline Cnt frexp(Cnt a, int* e) { return Cnt(::frexp(a.v, e)); }
+inline CntF frexp(CntF a, int* e) { return CntF(::frexpf(a.v, e)); }
+inline bool signbit(Cnt a) { return std::signbit(a.v); }
+inline bool signbit(CntF a) { return std::signbit(a.v); }
+inline bool isfinite(Cnt a) { return std::isfinite(a.v); }
+inline bool isfinite(CntF a) { return std::isfinite(a.v); }
 inline Cnt inv_sqrt(Cnt a) { g_ops64++; return Cnt(1.0 / ::sqrt(a.v)); }
 inline CntF inv_sqrt(CntF a) { g_ops32++; return CntF(1.0f / ::sqrtf(a.v)); }
 

@@ -49,7 +49,8 @@ def opcounts():
     """Operation counts of the GPU algorithm (assist_amd/csrc/opcount.cpp)."""
     exe = os.path.join(ROOT, "assist_amd", "_build", "opcount")
     src = os.path.join(ROOT, "assist_amd", "csrc", "opcount.cpp")
-    if not os.path.exists(exe) or os.path.getmtime(exe) < os.path.getmtime(src):
+    model = os.path.join(ROOT, "assist_amd", "csrc", "force_model.h")
+    if not os.path.exists(exe) or os.path.getmtime(exe) < max(os.path.getmtime(src), os.path.getmtime(model)):
         os.makedirs(os.path.dirname(exe), exist_ok=True)
         subprocess.run(["g++", "-O1", "-std=c++17", "-o", exe, src], check=True)
     return json.loads(subprocess.run([exe], check=True, capture_output=True, text=True).stdout)

@@ -19,11 +19,11 @@ FORCE_TERMS = {
 }
 
 
-def _forces_both(ephem, oracle, t, states, params, forces, precision=0):
+def _forces_both(ephem, oracle, t, states, params, forces, precision=0, **shape):
     from assist_amd import batch
-    opts = dict(forces=forces)
+    opts = dict(forces=forces, **shape)
     g, gs = batch.forces(ephem, t, states, params, extras=opts, precision=precision)
-    o, os_ = oracle.forces(t, states, params, oracle.opts(forces=forces))
+    o, os_ = oracle.forces(t, states, params, oracle.opts(forces=forces, **shape))
     assert (gs == os_).all()
     return g, o
 
@@ -90,6 +90,31 @@ def test_forces_mixed_precision_error(gpu, ephem, oracle, model):
     assert e_var <= 1e-5
 
 
+# Marsden g(r) = alpha (r/r0)^-nm (1 + (r/r0)^nn)^-nk: the default 1/r^2, the water-ice law of
+# Marsden, Sekanina & Yeomans (1973), and another nk = 0 shape (the kernels skip the powers that
+# nk = 0 makes exactly 1 or -0)
+MARSDEN_SHAPES = {
+    "default": {},
+    "comet": dict(alpha=0.1112620426, nk=4.6142, nm=2.15, nn=5.093, r0=2.808),
+    "nk0": dict(alpha=0.5, nk=0.0, nm=2.6, nn=3.0, r0=1.5),
+}
+
+
+@pytest.mark.parametrize("shape", list(MARSDEN_SHAPES))
+def test_forces_marsden_shapes(gpu, ephem, oracle, model, shape):
+    t, states, params = _sample(model, 256, 15, ng=1.0)
+    kw = MARSDEN_SHAPES[shape]
+    g, o = _forces_both(ephem, oracle, t, states, params, 0x08, **kw)
+    e_term = rel_err(g, o).max()
+    g, o = _forces_both(ephem, oracle, t, states, params, 0x7F, **kw)
+    e_full = rel_err(g, o).max()
+    gm, om = _forces_both(ephem, oracle, t, states, params, 0x7F, precision=1, **kw)
+    e_mixed = rel_err(gm[:, 0], om[:, 0]).max()
+    print("Marsden %s: term %.3e, full model %.3e, mixed real %.3e" % (shape, e_term, e_full, e_mixed))
+    assert e_term <= 1e-12 and e_full <= 1e-12
+    assert e_mixed <= 1e-11
+
+
 def _integrate_both(ephem, oracle, t0, t1, states, params=None, **cfg):
     from assist_amd import batch
     r = batch.integrate(ephem, t0, t1, states, params, config=batch.BatchConfig.default(**cfg))
@@ -133,6 +158,24 @@ def test_integrate_uneven_spans_both_directions(gpu, ephem, oracle, model):
     print("uneven: max |dr| main belt %.3e m, NEO %.3e m" % (dpos[~is_neo].max(), dpos[is_neo].max()))
     assert dpos[~is_neo].max() <= 1.0
     assert np.median(dpos[is_neo]) <= 1.0
+
+
+@pytest.mark.parametrize("shape", ["comet", "nk0"])
+def test_integrate_marsden_shapes(gpu, ephem, oracle, model, shape):
+    """Main-belt orbits with strong non-gravitational forces of a non-default shape, 1000 days."""
+    from assist_amd import batch
+    n = 32
+    states, params, _ = population(model, n, 16, n_var=6, ng_fraction=1.0, neo_fraction=0.0)
+    params[:, 0] = np.random.default_rng(16).normal(size=(n, 3)) * 1e-10
+    kw = MARSDEN_SHAPES[shape]
+    r = batch.integrate(ephem, 0.0, 1000.0, states, params, extras=dict(forces=0x7F, **kw))
+    o = oracle.integrate(0.0, 1000.0, states, params, opts=oracle.opts(forces=0x7F, **kw))
+    assert (r.status == 0).all() and (o["status"] == 0).all()
+    dpos = np.linalg.norm(r.states[:, 0, :3] - o["states"][:, 0, :3], axis=-1) * AU_M
+    dvar = rel_err(r.states[:, 1:], o["states"][:, 1:])
+    print("Marsden %s, 1000 d: max |dr| = %.3e m, var rel %.3e" % (shape, dpos.max(), dvar.max()))
+    assert dpos.max() <= 1.0
+    assert dvar.max() <= 1e-6
 
 
 @pytest.mark.parametrize("mode,span,tol_m", [

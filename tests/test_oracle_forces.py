@@ -9,7 +9,8 @@ independent statements of the physics:
   Williams 1983, beta = gamma = 1) with the Sun as the only source;
 * simple GR: the 1PN Schwarzschild test-particle acceleration; potential GR:
   -6 GM^2 r / (c^2 r^4);
-* Marsden forces with the default g(r) = 1/r^2;
+* Marsden forces with the default g(r) = 1/r^2 and with a comet-like shape
+  g(r) = alpha (r/r0)^-nm (1 + (r/r0)^nn)^-nk (Marsden, Sekanina & Yeomans 1973);
 * every term's variational partials: central finite differences of the real
   acceleration (positions, velocities and the Marsden A's);
 * reference quirks: the variational point-mass term ignores the force flags
@@ -195,7 +196,7 @@ def test_marsden_default_shape(oracle, model):
     assert (a0 == 0).all()
 
 
-def _jacobian_cols(oracle, t, x, forces, params=None):
+def _jacobian_cols(oracle, t, x, forces, params=None, **opts):
     """Variational accelerations for unit variations of x, y, z, vx, vy, vz (and A1..A3)."""
     n = len(t)
     nv = 6 if params is None else 9
@@ -209,9 +210,44 @@ def _jacobian_cols(oracle, t, x, forces, params=None):
         pp[:, 0] = params
         for k in range(3):
             pp[:, 7 + k, k] = 1.0
-    acc, s = oracle.forces(t, st, pp, oracle.opts(forces=forces))
+    acc, s = oracle.forces(t, st, pp, oracle.opts(forces=forces, **opts))
     assert (s == 0).all()
     return acc[:, 1:]
+
+
+# water-ice sublimation law of Marsden, Sekanina & Yeomans (1973)
+COMET_SHAPE = dict(alpha=0.1112620426, nk=4.6142, nm=2.15, nn=5.093, r0=2.808)
+
+
+def test_marsden_comet_shape(oracle, model):
+    t, x = _states(model, 12, 12)
+    rng = np.random.default_rng(12)
+    A = rng.normal(size=(12, 3)) * 1e-9
+    a = _acc(oracle, t, x, 0x08, A, **COMET_SHAPE)
+    c = COMET_SHAPE
+    for i in range(len(t)):
+        s = _body(oracle, 0, t[i])
+        r, v = x[i, :3] - s[1:4], x[i, 3:] - s[4:7]
+        rn = np.linalg.norm(r)
+        q = rn / c["r0"]
+        g = c["alpha"] * q ** -c["nm"] * (1 + q ** c["nn"]) ** -c["nk"]
+        h = np.cross(r, v)
+        tt = np.cross(h, r)
+        ref = g * (A[i, 0] * r / rn + A[i, 1] * tt / np.linalg.norm(tt) + A[i, 2] * h / np.linalg.norm(h))
+        assert np.linalg.norm(a[i] - ref) <= 1e-13 * np.linalg.norm(ref)
+    # partials of the shaped term, including dg/dr's (1 + q^nn) factor, by central differences
+    params = rng.normal(size=(4, 3)) * 1e-9
+    t, x = t[:4], x[:4]
+    cols = _jacobian_cols(oracle, t, x, 0x08, params, **COMET_SHAPE)
+    for i in range(4):
+        for k in range(6):
+            h = (1e-6 if k < 3 else 1e-8) * (np.linalg.norm(x[i, :3]) if k < 3 else np.linalg.norm(x[i, 3:]))
+            xp, xm = x[i].copy(), x[i].copy()
+            xp[k] += h
+            xm[k] -= h
+            ap = _acc(oracle, np.array([t[i]] * 2), np.stack([xp, xm]), 0x08, np.stack([params[i]] * 2), **COMET_SHAPE)
+            fd = (ap[0] - ap[1]) / (2 * h)
+            assert np.abs(cols[i, k] - fd).max() <= 1e-6 * np.abs(cols[i]).max(), (i, k)
 
 
 @pytest.mark.parametrize("term", ["DIRECT", "NON_GRAVITATIONAL", "EARTH_HARMONICS", "SUN_HARMONICS", "GR_EIH",
