@@ -140,7 +140,7 @@ ias15_kernel(IntegrateArgs args) {
     // br[7][3], er[7][3] of the last accepted step, per lane, in the scratch buffer
     double* scr = ka->scratch + ((size_t)grp * G + lane) * 42;
 
-    double x0[3], v0[3], csx[3], csv[3], a0[3];
+    double x0[3] = {0.0, 0.0, 0.0}, v0[3] = {0.0, 0.0, 0.0}, csx[3], csv[3], a0[3];
     // b and g (7 x 3 each per vector) live in LDS, one column per owning lane, to relieve
     // register pressure; lanes that own no vector share one spare column (their values are
     // never used), which is what lets 7-vector systems on 8-lane groups fit two blocks per CU.
@@ -186,7 +186,7 @@ ias15_kernel(IntegrateArgs args) {
     int sys = active ? (ka->order ? ka->order[grp] : grp) : 0;
     int cur = 0, cend = 0;  // dense output: next and end index of this system's output times
     constexpr bool kWaveRounds = ias15_cache_words(G, Ext) > 0;  // 8-lane groups
-    SubstepCache scache;
+    SubstepCache scache{};
     // the next trip is a later iteration at substep nn: load its cached positions now
     auto prefetch = [&](int nn) {
         if constexpr (kWaveRounds) {
@@ -253,6 +253,8 @@ ias15_kernel(IntegrateArgs args) {
         if (active) {
             sys = ka->order ? ka->order[next] : next;
             load(sys);
+        } else {
+            ng = false;  // (a helper group takes no Marsden branch)
         }
     };
     // Prepare an attempt of the current step: trim to t_end, predict b from the last
@@ -320,11 +322,16 @@ ias15_kernel(IntegrateArgs args) {
     };
 
     if (active) load(sys);
-    while (active) {
+    // 8-lane groups: a group whose queue ran dry stays in the loop as a helper while any group of
+    // its wave is active.  It joins each trip's force evaluation up to the wave-wide Chebyshev
+    // rounds, where its lanes take items of the groups that need fresh positions (the rounds of
+    // a thinned-out wave, e.g. the last long-span systems of an uneven batch, drop from 4 to 1),
+    // and leaves the trip right after the evaluation without touching any state or output.
+    while (kWaveRounds ? (__ballot(active) != 0ULL) : active) {
         ka = launder(ka);
         const DevEphem& E = *ka->E;
         // zero-length span (dense output: no output times left)
-        if (!phase && (ka->out_t ? cur >= cend : t == t_end)) { finish(); continue; }
+        if (active && !phase && (ka->out_t ? cur >= cend : t == t_end)) { finish(); continue; }
         // ---- evaluation point of this trip ----
         double xe[3], ve[3], te;
         if (!phase) {
@@ -354,6 +361,12 @@ ias15_kernel(IntegrateArgs args) {
             te = t + s[0];
         }
         // ---- force evaluation (whole group) ----
+        if (kWaveRounds && !active) {
+            // a helper evaluates at a fixed main-belt point: its results are discarded, and a
+            // common point keeps it on the branches the active groups take
+#pragma unroll
+            for (int c = 0; c < 3; c++) { xe[c] = (c == 0) ? 2.5 : 0.0; ve[c] = (c == 1) ? 0.01 : 0.0; }
+        }
         double xr[3], vr[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) { xr[c] = gbcast<G, 0>(xe[c]); vr[c] = gbcast<G, 0>(ve[c]); }
@@ -368,10 +381,11 @@ ias15_kernel(IntegrateArgs args) {
             scache.base = ka->ecache ? ka->ecache + (size_t)(tid >> 6) * (64 * ias15_cache_words(G, Ext)) : nullptr;
             scache.xch = s_xch;
             scache.n = n;
-            scache.mode = (!ka->ecache || !phase) ? kEcEval : (it == 1 ? kEcFill : kEcUse);
+            scache.mode = !active ? kEcUse : ((!ka->ecache || !phase) ? kEcEval : (it == 1 ? kEcFill : kEcUse));
             scp = &scache;
         }
         force_eval<G, Mixed, double, Ext>(E, s_slots, s_gm, fo, lane, te, xr, vr, A, ng, nv > 1, R, J, scp);
+        if (kWaveRounds && !active) continue;
         double at[3];
         own_accel<double, TS>(lane, R, J, xe, ve, dA, ng, at);
         evals++;
