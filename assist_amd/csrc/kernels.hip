@@ -140,7 +140,7 @@ ias15_kernel(IntegrateArgs args) {
     // br[7][3], er[7][3] of the last accepted step, per lane, in the scratch buffer
     double* scr = ka->scratch + ((size_t)grp * G + lane) * 42;
 
-    double x0[3] = {0.0, 0.0, 0.0}, v0[3] = {0.0, 0.0, 0.0}, csx[3], csv[3], a0[3];
+    double x0[3] = {2.5, 0.0, 0.0}, v0[3] = {0.0, 0.01, 0.0}, csx[3], csv[3], a0[3];
     // b and g (7 x 3 each per vector) live in LDS, one column per owning lane, to relieve
     // register pressure; lanes that own no vector share one spare column (their values are
     // never used), which is what lets 7-vector systems on 8-lane groups fit two blocks per CU.
@@ -254,7 +254,12 @@ ias15_kernel(IntegrateArgs args) {
             sys = ka->order ? ka->order[next] : next;
             load(sys);
         } else {
-            ng = false;  // (a helper group takes no Marsden branch)
+            // a helper group evaluates at a fixed main-belt point (its results are discarded): a
+            // common point keeps it on the branches the active groups take, and no Marsden branch
+            ng = false;
+            phase = false;
+#pragma unroll
+            for (int c = 0; c < 3; c++) { x0[c] = (c == 0) ? 2.5 : 0.0; v0[c] = (c == 1) ? 0.01 : 0.0; }
         }
     };
     // Prepare an attempt of the current step: trim to t_end, predict b from the last
@@ -361,12 +366,6 @@ ias15_kernel(IntegrateArgs args) {
             te = t + s[0];
         }
         // ---- force evaluation (whole group) ----
-        if (kWaveRounds && !active) {
-            // a helper evaluates at a fixed main-belt point: its results are discarded, and a
-            // common point keeps it on the branches the active groups take
-#pragma unroll
-            for (int c = 0; c < 3; c++) { xe[c] = (c == 0) ? 2.5 : 0.0; ve[c] = (c == 1) ? 0.01 : 0.0; }
-        }
         double xr[3], vr[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) { xr[c] = gbcast<G, 0>(xe[c]); vr[c] = gbcast<G, 0>(ve[c]); }
