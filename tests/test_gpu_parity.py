@@ -287,8 +287,8 @@ def test_integrate_is_deterministic(gpu, ephem, model):
     assert np.array_equal(outs[0][0], outs[1][0])
 
 
-@pytest.mark.parametrize("n_var,ng", [(6, 0.0), (0, 0.3)])
-def test_results_do_not_depend_on_wave_mates(gpu, ephem, model, n_var, ng):
+@pytest.mark.parametrize("n_var,ng,precision", [(6, 0.0, 0), (0, 0.3, 0), (6, 0.3, 1)])
+def test_results_do_not_depend_on_wave_mates(gpu, ephem, model, n_var, ng, precision):
     """8-lane groups whose queue ran dry stay in their wave as helpers and take over Chebyshev
     items of the groups still integrating.  A system integrated alone (seven helper groups in
     its wave from the start) or in a small group of systems ends bit for bit where it ends
@@ -299,7 +299,7 @@ def test_results_do_not_depend_on_wave_mates(gpu, ephem, model, n_var, ng):
     states, params, _ = population(model, n, 47, n_var=n_var, ng_fraction=ng)
     span = synthetic.log_uniform_spans(n, rng, 5.0, 600.0)
     t0 = np.zeros(n)
-    cfg = batch.BatchConfig.default(min_dt=0.001, lanes_per_system=8)
+    cfg = batch.BatchConfig.default(min_dt=0.001, lanes_per_system=8, precision=precision)
     full = batch.integrate(ephem, t0, span, states, params, config=cfg)
     assert (full.status == 0).all()
     for idx in ([int(np.argmax(span))], [0], [3, 7, 11]):
