@@ -794,14 +794,16 @@ static int integrate_device_impl(const struct assist_ephem* ephem, const struct 
         rc = tuned_shape(a, mixed, s, &shape);
         if (rc) return rc;
     }
-    // uneven batches: hand out the longest spans first (the order is identity for equal spans)
+    // large batches: hand out the systems of highest expected cost first (longest spans, smallest
+    // perihelia), so that the launch does not end on an expensive system handed out last
     if (n >= kTuneMinSystems && !dense && d_t1) {
         rc = ensure_workspace(0);
         if (rc) return rc;
         rc = ensure_order(n);
         if (rc) return rc;
-        if (!hip_ok(launch_span_order(n, d_t0, d_t1, g_ws.okeys, g_ws.okeys_alt, g_ws.ovals, g_ws.order, g_ws.otemp,
-                                      g_ws.otemp_bytes, s), "launch(span order)"))
+        if (!hip_ok(launch_span_order(a.E, n, a.nvec, d_t0, d_t1, a.states, g_ws.okeys, g_ws.okeys_alt, g_ws.ovals,
+                                      g_ws.order, g_ws.otemp, g_ws.otemp_bytes, s),
+                    "launch(queue order)"))
             return ASSIST_BATCH_ERROR_DEVICE;
         a.order = g_ws.order;
     }

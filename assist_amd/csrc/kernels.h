@@ -65,7 +65,7 @@ struct IntegrateArgs {
     const int* out_off;
     const double* out_t;
     double* out_states;
-    // queue order (optional): the q-th system handed out is order[q] (longest spans first)
+    // queue order (optional): the q-th system handed out is order[q] (longest span first)
     const int* order;
     // substep ephemeris cache of 8-lane groups (optional): [waves][8][7][kEcWords] doubles
     double* ecache;
@@ -110,12 +110,13 @@ hipError_t ias15_occupancy(int G, bool mixed, int waves, bool ext, int nvec, int
 inline bool ext_model(const ForceOpts& f) { return f.geocentric != 0 || ((f.forces & 0x40) && f.gr_eih_sources > 1); }
 size_t ias15_lds_bytes(int G, int nvec, int threads, int waves, bool ext);
 hipError_t launch_set_counter(int* q, int v, hipStream_t s);
-// Queue order for uneven batches: order[0..n) = system indices sorted by |t1 - t0|, longest first
-// (stable, so equal spans keep their index order).  Buffers: keys/keys_alt n words, vals n ints.
+// Queue order of large batches: order[0..n) = system indices sorted by span |t1 - t0|, longest
+// first, and among equal spans by the real particle's perihelion distance, smallest first (stable,
+// so equal keys keep their index order).  Buffers: keys/keys_alt n words, vals n ints.
 size_t span_order_temp_bytes(int n);
-hipError_t launch_span_order(int n, const double* t0, const double* t1, unsigned long long* keys,
-                             unsigned long long* keys_alt, int* vals, int* order, void* temp, size_t temp_bytes,
-                             hipStream_t s);
+hipError_t launch_span_order(const DevEphem* E, int n, int nvec, const double* t0, const double* t1,
+                             const double* states, unsigned long long* keys, unsigned long long* keys_alt, int* vals,
+                             int* order, void* temp, size_t temp_bytes, hipStream_t s);
 hipError_t launch_ephem_query(const DevEphem* E, int n, const int* body, const double* t, double* out, int* status,
                               hipStream_t s);
 

@@ -766,13 +766,37 @@ hipError_t launch_ephem_query(const DevEphem* E, int n, const int* body, const d
 // Queue order of uneven batches: systems sorted by span, longest first, so that the persistent
 // kernel starts the long integrations early and its tail is not set by one started late.
 // ------------------------------------------------------------------------------------------
-__global__ void span_keys_kernel(int n, const double* __restrict__ t0, const double* __restrict__ t1,
+// Queue keys: the span |t1 - t0| first, then 1/q, q the perihelion distance (au) of the real
+// particle's osculating orbit about the solar-system barycentre with the Sun's GM (the Sun sits
+// within 0.01 au of the barycentre).  IAS15 (eps 1e-9) takes about 7 + 0.15 |t1 - t0| / q steps on
+// main-belt and NEO-like orbits (a fit over 800-10 000-day spans, 11% rms in log), but a close
+// approach can multiply that, so the longest spans go first whatever their orbits, and among
+// equal spans the smallest perihelia.  The key is the span's bit pattern with its low 20 bits
+// (below 2^-32 of the span) replaced by min(1/q, 1000) x 1000.  Only the order of the queue
+// depends on the key.
+__global__ void span_keys_kernel(const DevEphem* __restrict__ E, int n, int nvec, const double* __restrict__ t0,
+                                 const double* __restrict__ t1, const double* __restrict__ states,
                                  unsigned long long* __restrict__ keys, int* __restrict__ vals) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double span = fabs(t1[i] - t0[i]);
+    const double* st = states + (size_t)i * nvec * 6;
+    const double mu = E->GM[0];
+    const double x = st[0], y = st[1], z = st[2], vx = st[3], vy = st[4], vz = st[5];
+    const double r = sqrt(x * x + y * y + z * z);
+    const double v2 = vx * vx + vy * vy + vz * vz;
+    const double hx = y * vz - z * vy, hy = z * vx - x * vz, hz = x * vy - y * vx;
+    const double h2 = hx * hx + hy * hy + hz * hz;
+    const double inv_a = 2.0 / r - v2 / mu;
+    double q = r;
+    if (inv_a > 0.0) {
+        const double e2 = fmax(0.0, 1.0 - h2 * inv_a / mu);
+        q = (1.0 - sqrt(e2)) / inv_a;
+    }
+    const double rate = (q == q) ? fmin(1.0 / fmax(q, 1e-3), 1000.0) : 0.0;
+    const unsigned long long low = (unsigned long long)(rate * 1000.0);  // < 2^20
     // non-negative doubles sort like their bit patterns; NaN sorts last
-    keys[i] = (span == span) ? (unsigned long long)__double_as_longlong(span) : 0ull;
+    keys[i] = (span == span) ? (((unsigned long long)__double_as_longlong(span) & ~0xFFFFFull) | low) : 0ull;
     vals[i] = i;
 }
 
@@ -784,11 +808,12 @@ size_t span_order_temp_bytes(int n) {
     return bytes;
 }
 
-hipError_t launch_span_order(int n, const double* t0, const double* t1, unsigned long long* keys,
-                             unsigned long long* keys_alt, int* vals, int* order, void* temp, size_t temp_bytes,
-                             hipStream_t s) {
+hipError_t launch_span_order(const DevEphem* E, int n, int nvec, const double* t0, const double* t1,
+                             const double* states, unsigned long long* keys, unsigned long long* keys_alt, int* vals,
+                             int* order, void* temp, size_t temp_bytes, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(span_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, t0, t1, keys, vals);
+    hipLaunchKernelGGL(span_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, s, E, n, nvec, t0, t1, states, keys,
+                       vals);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys, keys_alt, vals, order, n, 0, 64, s);
