@@ -763,8 +763,8 @@ hipError_t launch_ephem_query(const DevEphem* E, int n, const int* body, const d
 }
 
 // ------------------------------------------------------------------------------------------
-// Queue order of uneven batches: systems sorted by span, longest first, so that the persistent
-// kernel starts the long integrations early and its tail is not set by one started late.
+// Queue order of large batches: systems sorted by expected length, longest first, so that the
+// persistent kernel starts the long integrations early and its tail is not set by one started late.
 // ------------------------------------------------------------------------------------------
 // Queue keys: the span |t1 - t0| first, then 1/q, q the perihelion distance (au) of the real
 // particle's osculating orbit about the solar-system barycentre with the Sun's GM (the Sun sits
