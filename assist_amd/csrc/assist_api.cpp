@@ -130,13 +130,13 @@ int assist_ephem_init(struct assist_ephem* ephem, char* user_planets_path, char*
     } else {
         snprintf(asteroids_path, FN, "%s", user_asteroids_path);
     }
-    if (!ast_missing && !(ephem->spk_asteroids = assist_host::spk_init(asteroids_path))) ast_missing = 1;
+    if (!ast_missing && !(ephem->spk_asteroids = assist_spk_init(asteroids_path))) ast_missing = 1;
     if (fmt == FILE_FORMAT_ASCII_BIN) {
         // planets from a JPL binary file: its constants table gives the force-model constants and
         // the asteroid masses (reference src/assist.c:291-327)
         ephem->ascii_planets = assist_ascii_init(planets_path);
         if (!ephem->ascii_planets) {
-            if (ephem->spk_asteroids) { assist_host::spk_free(ephem->spk_asteroids); ephem->spk_asteroids = nullptr; }
+            if (ephem->spk_asteroids) { assist_spk_free(ephem->spk_asteroids); ephem->spk_asteroids = nullptr; }
             return ASSIST_ERROR_EPHEM_FILE;
         }
         const ascii_s* a = ephem->ascii_planets;
@@ -159,7 +159,8 @@ int assist_ephem_init(struct assist_ephem* ephem, char* user_planets_path, char*
         ephem->over_c_squared = 1.0 / ephem->c_squared;
         ephem->planets_calc = ascii_planets_calc_cb;
         if (ephem->spk_asteroids) {
-            for (auto& t : ephem->spk_asteroids->targets) {
+            for (int i = 0; i < ephem->spk_asteroids->num; i++) {
+                spk_target& t = ephem->spk_asteroids->targets[i];
                 if (t.code >= 2000000) {
                     char key[16];  // only the first 6 characters take part in the match
                     snprintf(key, sizeof key, "MA%04d", t.code - 2000000);
@@ -180,35 +181,24 @@ int assist_ephem_init(struct assist_ephem* ephem, char* user_planets_path, char*
     if (fmt != FILE_FORMAT_VALID_BSP) {
         fprintf(stderr, "(ASSIST) Error: Failed to initialize planets ephemeris from '%s'.\n", planets_path);
         fprintf(stderr, "(ASSIST) Supported planet files: NAIF SPK kernels (.bsp) and JPL binary ephemerides (.440/.441).\n");
-        if (ephem->spk_asteroids) { assist_host::spk_free(ephem->spk_asteroids); ephem->spk_asteroids = nullptr; }
+        if (ephem->spk_asteroids) { assist_spk_free(ephem->spk_asteroids); ephem->spk_asteroids = nullptr; }
         return ASSIST_ERROR_EPHEM_FILE;
     }
-    ephem->spk_planets = assist_host::spk_init(planets_path);
+    ephem->spk_planets = assist_spk_init(planets_path);
     if (!ephem->spk_planets) {
-        if (ephem->spk_asteroids) { assist_host::spk_free(ephem->spk_asteroids); ephem->spk_asteroids = nullptr; }
+        if (ephem->spk_asteroids) { assist_spk_free(ephem->spk_asteroids); ephem->spk_asteroids = nullptr; }
         return ASSIST_ERROR_EPHEM_FILE;
     }
     ephem->planets_source = FILE_FORMAT_VALID_BSP;
     static const int naif[ASSIST_BODY_NPLANETS] = {10, 1, 2, 399, 301, 4, 5, 6, 7, 8, 9};
     for (int k = 0; k < ASSIST_BODY_NPLANETS; k++) ephem->spk_target_index[k] = assist_host::find_target(ephem->spk_planets, naif[k]);
     ephem->spk_emb_index = assist_host::find_target(ephem->spk_planets, 3);
-    spk_constants c = assist_host::load_constants(planets_path);
-    ephem->AU = c.AU;
-    ephem->EMRAT = c.EMRAT;
-    ephem->J2E = c.J2E;
-    ephem->J3E = c.J3E;
-    ephem->J4E = c.J4E;
-    ephem->J2SUN = c.J2SUN;
-    ephem->RE = c.RE;
-    ephem->CLIGHT = c.CLIGHT;
-    ephem->ASUN = c.ASUN;
-    ephem->Re_eq = ephem->RE / ephem->AU;
-    ephem->Rs_eq = ephem->ASUN / ephem->AU;
-    ephem->c_AU_per_day = (ephem->CLIGHT / ephem->AU) * 86400;
-    ephem->c_squared = ephem->c_AU_per_day * ephem->c_AU_per_day;
-    ephem->over_c_squared = 1.0 / ephem->c_squared;
-    assist_host::join_masses(ephem->spk_planets, c, ephem->EMRAT);
-    assist_host::join_masses(ephem->spk_asteroids, c, ephem->EMRAT);
+    // constants and masses from the planets file's comment area (reference src/assist.c:330-345)
+    struct spk_constants_and_masses c = assist_load_spk_constants_and_masses(planets_path);
+    assist_apply_spk_constants(ephem, &c);
+    assist_spk_join_masses(ephem->spk_planets, &c.masses, ephem->EMRAT);
+    assist_spk_join_masses(ephem->spk_asteroids, &c.masses, ephem->EMRAT);
+    assist_free_spk_constants_and_masses(&c);
     ephem->planets_calc = planets_calc_cb;
     if (ast_missing) fprintf(stderr, "(ASSIST) %s\n", assist_error_messages[ASSIST_ERROR_AST_FILE]);
     return ASSIST_SUCCESS;
@@ -228,8 +218,8 @@ struct assist_ephem* assist_ephem_create(char* planets_file_name, char* asteroid
 
 void assist_ephem_free_pointers(struct assist_ephem* ephem) {
     if (!ephem) return;
-    if (ephem->spk_planets) assist_host::spk_free(ephem->spk_planets);
-    if (ephem->spk_asteroids) assist_host::spk_free(ephem->spk_asteroids);
+    if (ephem->spk_planets) assist_spk_free(ephem->spk_planets);
+    if (ephem->spk_asteroids) assist_spk_free(ephem->spk_asteroids);
     if (ephem->ascii_planets) assist_ascii_free(ephem->ascii_planets);
     ephem->spk_planets = nullptr;
     ephem->spk_asteroids = nullptr;
@@ -251,9 +241,9 @@ void assist_ephem_time_bounds(const struct assist_ephem* ephem, double* t_beg, d
     const spk_s* files[2] = {ephem->spk_planets, ephem->spk_asteroids};
     for (const spk_s* s : files) {
         if (!s) continue;
-        for (const auto& t : s->targets) {
-            if (t.beg > beg) beg = t.beg;
-            if (t.end < end) end = t.end;
+        for (int i = 0; i < s->num; i++) {
+            if (s->targets[i].beg > beg) beg = s->targets[i].beg;
+            if (s->targets[i].end < end) end = s->targets[i].end;
         }
     }
     if (t_beg) *t_beg = beg - ephem->jd_ref;

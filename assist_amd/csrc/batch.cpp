@@ -63,7 +63,7 @@ bool flatten_target(const spk_s* s, const spk_target& t, double jd_ref, std::vec
     const double* base = (const double*)s->map;
     double intlen = 0, init0 = 0;
     int rsize = 0, nrec_total = 0;
-    for (size_t k = 0; k < t.two.size(); k++) {
+    for (int k = 0; k < assist_host::segments(t); k++) {
         const double* dir = base + t.two[k] - 1;  // -> N
         const double init = dir[-3], len = dir[-2];
         const int R = (int)dir[-1], N = (int)dir[0];
@@ -79,7 +79,7 @@ bool flatten_target(const spk_s* s, const spk_target& t, double jd_ref, std::vec
     offset_words = out.size();
     out.resize(out.size() + (size_t)nrec_total * kRecWords, 0.0);
     double* dst = out.data() + offset_words;
-    for (size_t k = 0; k < t.two.size(); k++) {
+    for (int k = 0; k < assist_host::segments(t); k++) {
         const double* dir = base + t.two[k] - 1;
         const int N = (int)dir[0];
         const double* src = base + t.one[k] - 1;
@@ -145,7 +145,7 @@ int get_device_ephem(const assist_ephem* e, const DevEphem** out, DevEphem* host
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return ASSIST_BATCH_ERROR_DEVICE;
     spk_s* pl = e->spk_planets;
     ascii_s* jpl = pl ? nullptr : e->ascii_planets;
-    DeviceEphemCache* c = pl ? pl->gpu : (DeviceEphemCache*)jpl->device_cache;
+    DeviceEphemCache* c = (DeviceEphemCache*)(pl ? pl->device_cache : jpl->device_cache);
     if (c && c->device == dev && c->jd_ref == e->jd_ref && c->asteroids == e->spk_asteroids) {
         *out = c->d_ephem;
         if (host_copy) *host_copy = c->host;
@@ -153,7 +153,7 @@ int get_device_ephem(const assist_ephem* e, const DevEphem** out, DevEphem* host
     }
     if (c) {
         free_device_ephem(c);
-        if (pl) pl->gpu = nullptr;
+        if (pl) pl->device_cache = nullptr;
         else jpl->device_cache = nullptr;
     }
     static const int planet_codes[12] = {10, 1, 2, 3, 399, 301, 4, 5, 6, 7, 8, 9};
@@ -208,7 +208,20 @@ int get_device_ephem(const assist_ephem* e, const DevEphem** out, DevEphem* host
     H.inv_au = 1.0 / au_km;
     H.inv_au_ast = 1.0 / 149597870.7;
     H.vel_scale = 86400.0 / au_km;
-    assist_ephem_time_bounds(e, &H.t_beg, &H.t_end);
+    // coverage windows: every planet-file target, and every asteroid target
+    {
+        double pb = -INFINITY, pe = INFINITY, ab = -INFINITY, ae = INFINITY;
+        if (jpl) { pb = jpl->beg; pe = jpl->end; }
+        else for (int i = 0; i < pl->num; i++) { pb = fmax(pb, pl->targets[i].beg); pe = fmin(pe, pl->targets[i].end); }
+        for (int i = 0; i < n_ast; i++) {
+            ab = fmax(ab, e->spk_asteroids->targets[i].beg);
+            ae = fmin(ae, e->spk_asteroids->targets[i].end);
+        }
+        H.t_beg = pb - e->jd_ref;
+        H.t_end = pe - e->jd_ref;
+        H.ta_beg = n_ast > 0 ? ab - e->jd_ref : H.t_beg;
+        H.ta_end = n_ast > 0 ? ae - e->jd_ref : H.t_end;
+    }
     H.n_ast = n_ast;
     H.J2E = e->J2E; H.J3E = e->J3E; H.J4E = e->J4E; H.J2SUN = e->J2SUN;
     H.Re_eq = e->Re_eq; H.Rs_eq = e->Rs_eq; H.c_squared = e->c_squared; H.over_c_squared = e->over_c_squared;
@@ -235,7 +248,7 @@ int get_device_ephem(const assist_ephem* e, const DevEphem** out, DevEphem* host
         return ASSIST_BATCH_ERROR_DEVICE;
     }
     c->host = H;
-    if (pl) pl->gpu = c;
+    if (pl) pl->device_cache = c;
     else jpl->device_cache = c;
     *out = c->d_ephem;
     if (host_copy) *host_copy = H;
@@ -414,6 +427,12 @@ void pack(const reb_simulation* sim, const assist_extras* ax, const SimLayout& L
 }  // namespace
 
 // ---- launch of the integration kernel and its autotuner ----------------------------------
+namespace {
+
+// The workspace, the graph cache and the tuner's choices are process-wide: the integration entry
+// points hold this lock from the shape choice to the kernel's completion, so that calls from
+// several host threads run one after another instead of sharing one work queue and scratch.
+std::mutex g_launch_mutex;
 
 struct LaunchShape {
     int G = 8;         // lanes per system
@@ -561,6 +580,12 @@ int tuned_shape(const IntegrateArgs& base, bool mixed, hipStream_t s, LaunchShap
     a.t_out = nullptr;
     a.trace = nullptr;
     a.trace_count = nullptr;
+    a.snap_in = nullptr;
+    a.snap_out = nullptr;
+    a.t_pause = nullptr;
+    a.archive = nullptr;
+    a.archive_count = nullptr;
+    a.archive_cap = 0;
     LaunchShape best;
     best.G = lanes_for(base.nvec, 0);
     float best_ms = 1e30f;
@@ -599,6 +624,8 @@ int tuned_shape(const IntegrateArgs& base, bool mixed, hipStream_t s, LaunchShap
     return ASSIST_SUCCESS;
 }
 
+}  // namespace
+
 static int integrate_device_impl(const struct assist_ephem* ephem, const struct assist_extras* opts,
                                  const struct assist_batch_config* cfg_in, int n, int n_var, const double* d_t0,
                                  const double* d_t1, double* d_states, const double* d_params, int* d_status,
@@ -609,6 +636,7 @@ extern "C" {
 
 void assist_batch_config_default(struct assist_batch_config* cfg) {
     memset(cfg, 0, sizeof *cfg);  // every field, including those added later, starts defined
+    cfg->size = sizeof *cfg;
     cfg->epsilon = 1e-9;
     cfg->min_dt = 0.0;
     cfg->dt_initial = 0.001;
@@ -617,6 +645,12 @@ void assist_batch_config_default(struct assist_batch_config* cfg) {
     cfg->lanes_per_system = 0;
     cfg->block_size = 0;
     cfg->blocks_per_cu = 0;
+}
+
+void assist_batch_extra_init(struct assist_batch_extra* ex) {
+    memset(ex, 0, sizeof *ex);
+    ex->size = sizeof *ex;
+    ex->trace_every = 1;
 }
 
 const char* assist_batch_error_string(int code) {
@@ -718,6 +752,12 @@ int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const str
 
 }  // extern "C"
 
+static int integrate_device_locked(const struct assist_ephem* ephem, const struct assist_extras* opts,
+                                   const struct assist_batch_config& cfg, int n, int n_var, const double* d_t0,
+                                   const double* d_t1, double* d_states, const double* d_params, int* d_status,
+                                   long long* d_steps, long long* d_evals, long long* d_rejected, void* stream,
+                                   struct assist_batch_stats* stats, const struct assist_batch_extra* ex);
+
 // The device entry point with a per-system rejection count (d_rejected, or nullptr).
 static int integrate_device_impl(const struct assist_ephem* ephem, const struct assist_extras* opts,
                                  const struct assist_batch_config* cfg_in, int n, int n_var, const double* d_t0,
@@ -725,15 +765,30 @@ static int integrate_device_impl(const struct assist_ephem* ephem, const struct 
                                  long long* d_steps, long long* d_evals, long long* d_rejected, void* stream,
                                  struct assist_batch_stats* stats, const struct assist_batch_extra* ex) {
     const bool dense = ex && ex->out_t;
+    if (cfg_in && cfg_in->size != sizeof(struct assist_batch_config)) return ASSIST_BATCH_ERROR_ARG;
+    if (ex && ex->size != sizeof(struct assist_batch_extra)) return ASSIST_BATCH_ERROR_ARG;
     if (n < 0 || n_var < 0 || n_var > 7 || !d_t0 || (!d_t1 && !dense) || !d_states || !d_status)
         return ASSIST_BATCH_ERROR_ARG;
     if (ex && ex->trace && (!ex->trace_count || ex->trace_capacity < 0 || ex->trace_every < 1 || ex->trace_skip < 0))
         return ASSIST_BATCH_ERROR_ARG;
     if (dense && (!ex->out_offset || !ex->out_states)) return ASSIST_BATCH_ERROR_ARG;
+    if (ex && ex->archive && (!ex->archive_count || ex->archive_capacity < 1)) return ASSIST_BATCH_ERROR_ARG;
+    if (dense && ex->snapshot_in) return ASSIST_BATCH_ERROR_UNSUPPORTED;
     struct assist_batch_config cfg;
     if (cfg_in) cfg = *cfg_in;
     else assist_batch_config_default(&cfg);
     if (n == 0) return ASSIST_SUCCESS;
+    std::lock_guard<std::mutex> lock(g_launch_mutex);
+    return integrate_device_locked(ephem, opts, cfg, n, n_var, d_t0, d_t1, d_states, d_params, d_status, d_steps,
+                                   d_evals, d_rejected, stream, stats, ex);
+}
+
+static int integrate_device_locked(const struct assist_ephem* ephem, const struct assist_extras* opts,
+                                   const struct assist_batch_config& cfg, int n, int n_var, const double* d_t0,
+                                   const double* d_t1, double* d_states, const double* d_params, int* d_status,
+                                   long long* d_steps, long long* d_evals, long long* d_rejected, void* stream,
+                                   struct assist_batch_stats* stats, const struct assist_batch_extra* ex) {
+    const bool dense = ex && ex->out_t;
     IntegrateArgs a;
     memset(&a, 0, sizeof a);
     int rc = make_force_opts(opts, &a.fo);
@@ -767,10 +822,25 @@ static int integrate_device_impl(const struct assist_ephem* ephem, const struct 
         a.out_off = ex->out_offset;
         a.out_t = ex->out_t;
         a.out_states = ex->out_states;
+        a.snap_in = ex->snapshot_in;
+        a.snap_out = ex->snapshot_out;
+        a.t_pause = ex->t_pause;
+        a.archive = ex->archive;
+        a.archive_count = ex->archive ? ex->archive_count : nullptr;
+        a.archive_cap = ex->archive ? ex->archive_capacity : 0;
     }
     hipStream_t s = (hipStream_t)stream;
     LaunchShape shape;
-    shape.G = lanes_for(nv, cfg.lanes_per_system);
+    int lanes_req = cfg.lanes_per_system;
+    if (a.snap_in && lanes_req == 0) {
+        // a resumed batch continues with the lanes per system its snapshots were taken with (the
+        // one-lane and group kernels agree to rounding, not bit for bit)
+        double g = 0.0;
+        if (!hip_ok(hipMemcpy(&g, a.snap_in + 7, sizeof g, hipMemcpyDeviceToHost), "hipMemcpy(snapshot)"))
+            return ASSIST_BATCH_ERROR_DEVICE;
+        if (g == 1.0 || g == 2.0 || g == 4.0 || g == 8.0) lanes_req = (int)g;
+    }
+    shape.G = lanes_for(nv, lanes_req);
     if (shape.G < 0) return ASSIST_BATCH_ERROR_ARG;
     shape.threads = cfg.block_size > 0 ? cfg.block_size : kIas15Block;
     if (cfg.block_size == 0 && shape.G > 0) {
@@ -788,7 +858,7 @@ static int integrate_device_impl(const struct assist_ephem* ephem, const struct 
     // the extended force model (geocentric, several EIH sources) runs in FP64 with the 1-wave kernel
     const bool ext = ext_model(a.fo);
     if (ext && (mixed || shape.waves != 1)) return ASSIST_BATCH_ERROR_UNSUPPORTED;
-    const bool tune = cfg.lanes_per_system == 0 && cfg.block_size == 0 && cfg.waves_per_simd == 0 &&
+    const bool tune = lanes_req == 0 && cfg.block_size == 0 && cfg.waves_per_simd == 0 &&
                       n >= kTuneMinSystems && !dense && !ext;
     if (tune) {
         rc = tuned_shape(a, mixed, s, &shape);
@@ -796,7 +866,7 @@ static int integrate_device_impl(const struct assist_ephem* ephem, const struct 
     }
     // large batches: hand out the systems of highest expected cost first (longest spans, smallest
     // perihelia), so that the launch does not end on an expensive system handed out last
-    if (n >= kTuneMinSystems && !dense && d_t1) {
+    if (n >= kTuneMinSystems && !dense && d_t1 && !a.snap_in) {
         rc = ensure_workspace(0);
         if (rc) return rc;
         rc = ensure_order(n);
@@ -838,6 +908,48 @@ int assist_batch_get_particles_device(const struct assist_ephem* ephem, int n, c
     return ASSIST_SUCCESS;
 }
 
+int assist_batch_archive_interpolate_device(int n, int n_var, const double* d_archive, int capacity,
+                                            const int* d_count, int nq, const int* d_q_sys, const double* d_q_t,
+                                            double* d_out_states, int* d_status, void* stream) {
+    if (n < 0 || n_var < 0 || n_var > 7 || capacity < 1 || nq < 0) return ASSIST_BATCH_ERROR_ARG;
+    if (nq > 0 && (!d_archive || !d_count || !d_q_sys || !d_q_t || !d_out_states)) return ASSIST_BATCH_ERROR_ARG;
+    if (nq == 0) return ASSIST_SUCCESS;
+    hipStream_t s = (hipStream_t)stream;
+    if (!hip_ok(launch_archive_interp(n, nq, 1 + n_var, capacity, d_archive, d_count, d_q_sys, d_q_t, d_out_states,
+                                      d_status, s),
+                "launch(archive_interp)") ||
+        !hip_ok(hipStreamSynchronize(s), "archive_interp"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    return ASSIST_SUCCESS;
+}
+
+int assist_batch_archive_interpolate(int n, int n_var, const double* archive, int capacity, const int* count, int nq,
+                                     const int* q_sys, const double* q_t, double* out_states, int* status) {
+    if (n < 0 || n_var < 0 || n_var > 7 || capacity < 1 || nq < 0) return ASSIST_BATCH_ERROR_ARG;
+    if (nq > 0 && (!archive || !count || !q_sys || !q_t || !out_states)) return ASSIST_BATCH_ERROR_ARG;
+    if (nq == 0) return ASSIST_SUCCESS;
+    if (assist_gpu_device_count() <= 0) return ASSIST_BATCH_ERROR_DEVICE;
+    const int nv = 1 + n_var;
+    const size_t words = (size_t)n * capacity * ASSIST_SNAPSHOT_WORDS(nv), nqs = (size_t)nq;
+    DevBuf<double> da, dt, dout;
+    DevBuf<int> dc, dq, dst;
+    if (!da.alloc(words) || !dc.alloc((size_t)n) || !dq.alloc(nqs) || !dt.alloc(nqs) || !dout.alloc(nqs * nv * 6) ||
+        !dst.alloc(nqs))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    if (!hip_ok(hipMemcpy(da.p, archive, words * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy") ||
+        !hip_ok(hipMemcpy(dc.p, count, (size_t)n * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy") ||
+        !hip_ok(hipMemcpy(dq.p, q_sys, nqs * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy") ||
+        !hip_ok(hipMemcpy(dt.p, q_t, nqs * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    const int rc = assist_batch_archive_interpolate_device(n, n_var, da.p, capacity, dc.p, nq, dq.p, dt.p, dout.p,
+                                                           dst.p, nullptr);
+    if (rc) return rc;
+    if (!hip_ok(hipMemcpy(out_states, dout.p, nqs * nv * 6 * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy") ||
+        (status && !hip_ok(hipMemcpy(status, dst.p, nqs * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy")))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    return ASSIST_SUCCESS;
+}
+
 int assist_batch_get_particles(const struct assist_ephem* ephem, int n, const int* body, const double* t, double* out,
                                int* status) {
     if (n < 0 || (n > 0 && (!body || !t || !out))) return ASSIST_BATCH_ERROR_ARG;
@@ -862,6 +974,7 @@ int assist_batch_integrate(const struct assist_ephem* ephem, const struct assist
                            const struct assist_batch_config* cfg, int n, int n_var, const double* t0,
                            const double* t1, double* states, const double* params, int* status, long long* steps,
                            struct assist_batch_stats* stats) {
+    if (cfg && cfg->size != sizeof(struct assist_batch_config)) return ASSIST_BATCH_ERROR_ARG;
     if (n < 0 || n_var < 0 || n_var > 7 || !t0 || !t1 || !states) return ASSIST_BATCH_ERROR_ARG;
     if (n == 0) return ASSIST_SUCCESS;
     if (assist_gpu_device_count() <= 0) return ASSIST_BATCH_ERROR_DEVICE;
@@ -999,6 +1112,7 @@ int assist_batch_integrate_dense(const struct assist_ephem* ephem, const struct 
                                  const struct assist_batch_config* cfg, int n, int n_var, const double* t0,
                                  double* states, const double* params, const int* out_offset, const double* out_t,
                                  double* out_states, int* status, long long* steps, struct assist_batch_stats* stats) {
+    if (cfg && cfg->size != sizeof(struct assist_batch_config)) return ASSIST_BATCH_ERROR_ARG;
     if (n < 0 || n_var < 0 || n_var > 7 || !t0 || !states || !out_offset || !out_t || !out_states)
         return ASSIST_BATCH_ERROR_ARG;
     if (n == 0) return ASSIST_SUCCESS;
@@ -1030,8 +1144,7 @@ int assist_batch_integrate_dense(const struct assist_ephem* ephem, const struct 
         (params && !hip_ok(hipMemcpy(dp.p, params, ns * nv * 3 * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy")))
         return ASSIST_BATCH_ERROR_DEVICE;
     struct assist_batch_extra ex;
-    memset(&ex, 0, sizeof ex);
-    ex.trace_every = 1;
+    assist_batch_extra_init(&ex);
     ex.out_offset = doff.p;
     ex.out_t = dt.p;
     ex.out_states = dout.p;

@@ -54,8 +54,9 @@ struct DevEphem {
     double inv_au;        // 1 / AU [km]              (planets, reference src/spk.c:590)
     double inv_au_ast;    // 1 / 149597870.7          (asteroids, reference src/spk.c:470)
     double vel_scale;     // 86400 / AU               (km/s -> au/d)
-    double t_beg, t_end;  // common coverage, days relative to jd_ref
-    int n_ast;            // asteroids present (0 if no asteroid file)
+    double t_beg, t_end;    // coverage of every planet-file target, days relative to jd_ref
+    double ta_beg, ta_end;  // coverage of every asteroid target (the planet window when none)
+    int n_ast;              // asteroids present (0 if no asteroid file)
     int pad_;
     // force-model constants (reference src/spk.c:769-789)
     double J2E, J3E, J4E, J2SUN, Re_eq, Rs_eq, c_squared, over_c_squared;
@@ -89,6 +90,18 @@ enum SystemStatus : int {
     kNonFinite = -1,
     kMaxSteps = -2,
     kDtUnderflow = -3,
+    kPaused = -4,         // stopped at the first step boundary at or past its pause time
+    kBadSystem = 100,     // a query named no system of the batch (ASSIST_BATCH_ERROR_ARG)
 };
+
+// Snapshot of one system's IAS15 state at a step boundary (doubles): a header, then one block
+// per vector.  Header: t, the next trial step, the last accepted step (signed; 0 before the
+// first), steps, evaluations, rejections, status, lanes per system of the kernel that took it.
+// Vector block: x[3], v[3], their Kahan
+// compensations csx[3], csv[3], the acceleration at the start of the last accepted step a0[3],
+// that step's b[7][3] and the correction e[7][3] of its prediction (k-major: b[3k + c]).
+constexpr int kSnapHeader = 8;
+constexpr int kSnapVector = 57;
+ASSIST_HD constexpr int snap_words(int nvec) { return kSnapHeader + kSnapVector * nvec; }
 
 }  // namespace assist_gpu

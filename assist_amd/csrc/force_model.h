@@ -926,7 +926,14 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
     constexpr int NRV = Ext ? NR : 1;  // rounds with velocities and accelerations kept
     const int forces = O.forces;
     const double tv = value_of(t);
-    out.status = (tv >= E.t_beg && tv <= E.t_end) ? kOk : kCoverage;
+    // The reference reports coverage errors from its direct term (reference src/forces.c:315-321,
+    // 362-367), whose variational loop fetches the first asteroid whatever the force flags, and
+    // from the geocentric frame (:76-80).  The asteroid window therefore applies when a direct
+    // term is on and an asteroid file is loaded.  The planet window always applies: outside it
+    // the reference's other terms would use unset ephemeris values.
+    bool covered = tv >= E.t_beg && tv <= E.t_end;
+    if ((O.forces & (F_SUN | F_PLANETS | F_ASTEROIDS)) && E.n_ast > 0) covered = covered && tv >= E.ta_beg && tv <= E.ta_end;
+    out.status = covered ? kOk : kCoverage;
     const bool eih = (forces & F_GR_EIH) && O.gr_eih_sources > 0;
     const bool want_sun_vel = eih || ((forces & F_NONGRAV) && ng) || (forces & F_GR_SIMPLE);
 

@@ -2,43 +2,15 @@
 #pragma once
 #include <stddef.h>
 
-#include <string>
-#include <vector>
-
 #include "../../include/assist.h"
-
-struct spk_target {
-    int code;
-    int cen;
-    double mass;  // GM; 0 if no constant was found
-    double beg;   // JD of the first segment's start
-    double end;   // JD of the last segment's end
-    double res;   // length of the first segment in days
-    std::vector<int> one, two;  // 1-based word addresses of each segment
-};
+#include "../../include/spk.h"
 
 namespace assist_gpu { struct DeviceEphemCache; }
 
-struct spk_s {
-    std::vector<spk_target> targets;
-    int num = 0;
-    void* map = nullptr;
-    size_t len = 0;
-    assist_gpu::DeviceEphemCache* gpu = nullptr;  // device copy (planets file only)
-};
-
-struct spk_constants {
-    double AU = 0, EMRAT = 0, J2E = 0, J3E = 0, J4E = 0, J2SUN = 0, RE = 0, CLIGHT = 0, ASUN = 0;
-    std::vector<std::string> names;
-    std::vector<double> values;
-};
-
 namespace assist_host {
 
-spk_s* spk_init(const char* path);
-void spk_free(spk_s* s);
-spk_constants load_constants(const char* path);
-void join_masses(spk_s* s, const spk_constants& c, double emrat);
+// Number of segments of a target (its one/two arrays).
+inline int segments(const spk_target& t) { return t.ind + 1; }
 int find_target(const spk_s* s, int code);
 
 // Position / velocity / acceleration of an SPK target in km, km/s, km/s^2.

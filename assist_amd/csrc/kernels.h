@@ -69,6 +69,20 @@ struct IntegrateArgs {
     const int* order;
     // substep ephemeris cache of 8-lane groups (optional): [waves][8][7][kEcWords] doubles
     double* ecache;
+    // checkpoint and resume (optional), snapshots of snap_words(nvec) doubles per system:
+    // snap_in resumes each system from its snapshot (t0 and dt0 are then not read), snap_out
+    // receives each system's snapshot when it stops, and a system with t_pause stops (status
+    // kPaused) at the first step boundary at or past t_pause[s] in its direction
+    const double* snap_in;
+    double* snap_out;
+    const double* t_pause;
+    // step archive (optional): system s writes its snapshot at its start and after every
+    // accepted step to archive[s][k] (k < archive_cap; later ones are counted, not written),
+    // and the number of snapshots it took to archive_count[s]
+    double* archive;
+    int* archive_count;
+    int archive_cap;
+    int pad2_;
 };
 
 // Doubles of the substep ephemeris cache per launched thread (0 when the kernel keeps none):
@@ -117,6 +131,11 @@ size_t span_order_temp_bytes(int n);
 hipError_t launch_span_order(const DevEphem* E, int n, int nvec, const double* t0, const double* t1,
                              const double* states, unsigned long long* keys, unsigned long long* keys_alt, int* vals,
                              int* order, void* temp, size_t temp_bytes, hipStream_t s);
+// States at query times from step archives (IntegrateArgs::archive layout): query q asks for
+// system qsys[q] (0 <= qsys[q] < nsys, else status kBadSystem) at qt[q]; out[q][nvec][6];
+// status kCoverage outside the archived steps.
+hipError_t launch_archive_interp(int nsys, int nq, int nvec, int cap, const double* archive, const int* count,
+                                 const int* qsys, const double* qt, double* out, int* status, hipStream_t s);
 hipError_t launch_ephem_query(const DevEphem* E, int n, const int* body, const double* t, double* out, int* status,
                               hipStream_t s);
 

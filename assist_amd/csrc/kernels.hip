@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "device_types.h"
@@ -185,6 +186,7 @@ ias15_kernel(IntegrateArgs args) {
     bool active = grp < ka->nsys;
     int sys = active ? (ka->order ? ka->order[grp] : grp) : 0;
     int cur = 0, cend = 0;  // dense output: next and end index of this system's output times
+    int nrec = 0;           // step archive: snapshots this system has taken
     constexpr bool kWaveRounds = ias15_cache_words(G, Ext) > 0;  // 8-lane groups
     SubstepCache scache{};
     // the next trip is a later iteration at substep nn: load its cached positions now
@@ -194,6 +196,33 @@ ias15_kernel(IntegrateArgs args) {
                 scache.base = ka->ecache + (size_t)(tid >> 6) * (64 * ias15_cache_words(G, Ext));
                 ec_prefetch(scache, lane, nn);
             }
+        }
+    };
+
+    // Snapshot of the current system at a step boundary (layout in device_types.h): each lane
+    // writes its own vector's block, lane 0 the header.  Before the first step the b and e
+    // blocks are written as zeros (the lane's scratch then holds another system's).
+    auto write_snap = [&](double* base) {
+        if (own) {
+            double* w = base + kSnapHeader + kSnapVector * lane;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                w[c] = x0[c]; w[3 + c] = v0[c]; w[6 + c] = csx[c]; w[9 + c] = csv[c]; w[12 + c] = a0[c];
+            }
+            const bool stepped = dt_last != 0.0;
+            for (int k = 0; k < 42; k++) w[15 + k] = stepped ? scr[k] : 0.0;
+        }
+        if (lane == 0) {
+            base[0] = t; base[1] = dt; base[2] = dt_last;
+            base[3] = (double)steps; base[4] = (double)evals; base[5] = (double)rejected;
+            base[6] = (double)status; base[7] = (double)G;
+        }
+    };
+    auto archive_snap = [&]() {
+        if (ka->archive) {
+            if (nrec < ka->archive_cap)
+                write_snap(ka->archive + ((size_t)sys * ka->archive_cap + nrec) * snap_words(nv));
+            nrec++;
         }
     };
 
@@ -232,6 +261,28 @@ ias15_kernel(IntegrateArgs args) {
         status = 0;
         steps = evals = rejected = 0;
         last = 0;
+        if (ka->snap_in) {
+            // resume: the state, steps and counts of the snapshot, and the last step's b and e
+            // for the prediction of the next step (the first evaluation recomputes a0)
+            const double* sn = ka->snap_in + (size_t)s * snap_words(nv);
+            t = sn[0];
+            dt = sn[1];
+            dt_last = sn[2];
+            steps = (long long)sn[3];
+            evals = (long long)sn[4];
+            rejected = (long long)sn[5];
+            if (dt * (t_end - t) < 0) dt = -dt;
+            if (own) {
+                const double* w = sn + kSnapHeader + kSnapVector * lane;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    x0[c] = w[c]; v0[c] = w[3 + c]; csx[c] = w[6 + c]; csv[c] = w[9 + c]; a0[c] = w[12 + c];
+                }
+                for (int k = 0; k < 42; k++) scr[k] = w[15 + k];
+            }
+        }
+        nrec = 0;
+        archive_snap();
     };
     auto finish = [&]() {
         if (own) {
@@ -239,6 +290,8 @@ ias15_kernel(IntegrateArgs args) {
 #pragma unroll
             for (int c = 0; c < 3; c++) { st[c] = x0[c]; st[3 + c] = v0[c]; }
         }
+        if (ka->snap_out) write_snap(ka->snap_out + (size_t)sys * snap_words(nv));
+        if (lane == 0 && ka->archive_count) ka->archive_count[sys] = nrec;
         if (lane == 0) {
             if (ka->status) ka->status[sys] = status;
             if (ka->steps) ka->steps[sys] = steps;
@@ -587,7 +640,9 @@ ias15_kernel(IntegrateArgs args) {
         for (int c = 0; c < 3; c++) bad |= !(fabs(x0[c]) < 1e300) || !(fabs(v0[c]) < 1e300);
         bad = gor<G>(own ? bad : 0);
         if (bad) { status = kNonFinite; finish(); continue; }
+        archive_snap();
         if (last || (ka->out_t && cur >= cend)) { finish(); continue; }
+        if (ka->t_pause && (t - ka->t_pause[sys]) * dt_last >= 0) { status = kPaused; finish(); continue; }
         if (steps >= ka->io.max_steps) { status = kMaxSteps; finish(); continue; }
         phase = false;
     }
@@ -726,7 +781,10 @@ __global__ void __launch_bounds__(256) ephem_query_kernel(const DevEphem* __rest
     if (b < 0) st = 4;
     else if (b >= kNumPlanets && E.n_ast == 0) st = 2;
     else if (b >= kNumPlanets + E.n_ast) st = 3;
+    // a planet needs its own target's coverage; an asteroid its own and the Sun's, which makes
+    // its position barycentric (reference src/forces.c:213-217, src/spk.c:417-419)
     else if (!(tq >= E.t_beg && tq <= E.t_end)) st = kCoverage;
+    else if (b >= kNumPlanets && !(tq >= E.ta_beg && tq <= E.ta_end)) st = kCoverage;
     if (st != kOk) {
         for (int k = 0; k < 10; k++) o[k] = 0.0;
         if (status) status[q] = st;
@@ -759,6 +817,100 @@ hipError_t launch_ephem_query(const DevEphem* E, int n, const int* body, const d
                               hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(ephem_query_kernel, dim3((n + 255) / 256), dim3(256), 0, s, E, n, body, t, out, status);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// Interpolation in a step archive (the counterpart of reference src/assist.c:599-633, 682-752:
+// the state at t between two saved step boundaries, from the earlier one's state and the later
+// one's acceleration at the start of the step, b coefficients and step size).  One thread per
+// (query, vector).
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) archive_interp_kernel(int nsys, int nq, int nvec, int cap, const double* __restrict__ archive,
+                                                             const int* __restrict__ count, const int* __restrict__ qsys,
+                                                             const double* __restrict__ qt, double* __restrict__ out,
+                                                             int* __restrict__ status) {
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (long long)nq * nvec) return;
+    const int q = (int)(tid / nvec), v = (int)(tid % nvec);
+    const int sw = snap_words(nvec);
+    const int s = qsys[q];
+    const double tq = qt[q];
+    double* o = out + ((size_t)q * nvec + v) * 6;
+    if (s < 0 || s >= nsys) {
+        for (int c = 0; c < 6; c++) o[c] = 0.0;
+        if (v == 0 && status) status[q] = kBadSystem;
+        return;
+    }
+    int m = count[s];
+    if (m > cap) m = cap;
+    const double* rec0 = archive + (size_t)s * cap * sw;
+    // the last record at or before tq in the direction of integration (records are in step order)
+    int k = -1;
+    if (m >= 1) {
+        const double dir = (m >= 2 && rec0[(size_t)sw + 2] < 0.0) ? -1.0 : 1.0;
+        if ((tq - rec0[0]) * dir >= 0.0) {
+            int lo = 0, hi = m - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if ((tq - rec0[(size_t)mid * sw]) * dir >= 0.0) lo = mid;
+                else hi = mid - 1;
+            }
+            k = lo;
+        }
+    }
+    const double* a = k >= 0 ? rec0 + (size_t)k * sw : nullptr;
+    int st = kOk;
+    if (!a || (k == m - 1 && tq != a[0])) st = kCoverage;
+    if (st != kOk) {
+        for (int c = 0; c < 6; c++) o[c] = 0.0;
+        if (v == 0 && status) status[q] = st;
+        return;
+    }
+    const double* xa = a + kSnapHeader + kSnapVector * v;
+    if (tq == a[0]) {
+        for (int c = 0; c < 6; c++) o[c] = xa[c];
+    } else {
+        const double* bh = a + sw;  // the next record: the step from a to it
+        const double* xb = bh + kSnapHeader + kSnapVector * v;
+        const double dts = bh[2];
+        const double h = (tq - a[0]) / dts;
+        const double* a0 = xb + 12;
+        const double* b = xb + 15;
+        double s_[9], sv[8];
+        s_[0] = dts * h;
+        s_[1] = s_[0] * s_[0] / 2.;
+        s_[2] = s_[1] * h / 3.;
+        s_[3] = s_[2] * h / 2.;
+        s_[4] = 3. * s_[3] * h / 5.;
+        s_[5] = 2. * s_[4] * h / 3.;
+        s_[6] = 5. * s_[5] * h / 7.;
+        s_[7] = 3. * s_[6] * h / 4.;
+        s_[8] = 7. * s_[7] * h / 9.;
+        sv[0] = dts * h;
+        sv[1] = sv[0] * h / 2.;
+        sv[2] = 2. * sv[1] * h / 3.;
+        sv[3] = 3. * sv[2] * h / 4.;
+        sv[4] = 4. * sv[3] * h / 5.;
+        sv[5] = 5. * sv[4] * h / 6.;
+        sv[6] = 6. * sv[5] * h / 7.;
+        sv[7] = 7. * sv[6] * h / 8.;
+        for (int c = 0; c < 3; c++) {
+            o[c] = xa[c] + (s_[8] * b[18 + c] + s_[7] * b[15 + c] + s_[6] * b[12 + c] + s_[5] * b[9 + c] +
+                            s_[4] * b[6 + c] + s_[3] * b[3 + c] + s_[2] * b[c] + s_[1] * a0[c] + s_[0] * xa[3 + c]);
+            o[3 + c] = xa[3 + c] + sv[7] * b[18 + c] + sv[6] * b[15 + c] + sv[5] * b[12 + c] + sv[4] * b[9 + c] +
+                       sv[3] * b[6 + c] + sv[2] * b[3 + c] + sv[1] * b[c] + sv[0] * a0[c];
+        }
+    }
+    if (v == 0 && status) status[q] = kOk;
+}
+
+hipError_t launch_archive_interp(int nsys, int nq, int nvec, int cap, const double* archive, const int* count,
+                                 const int* qsys, const double* qt, double* out, int* status, hipStream_t s) {
+    if (nq <= 0) return hipSuccess;
+    const long long threads = (long long)nq * nvec;
+    hipLaunchKernelGGL(archive_interp_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, nsys, nq, nvec,
+                       cap, archive, count, qsys, qt, out, status);
     return hipGetLastError();
 }
 

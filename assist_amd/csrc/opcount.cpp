@@ -94,6 +94,8 @@ static DevEphem fake_ephem(std::vector<double>& buf) {
     E.inv_au_ast = 1.0 / 149597870.7;
     E.t_beg = -1e9;
     E.t_end = 1e9;
+    E.ta_beg = -1e9;
+    E.ta_end = 1e9;
     E.n_ast = 16;
     E.J2E = 1e-3; E.J3E = -2e-6; E.J4E = -1e-6; E.J2SUN = 2e-7; E.Re_eq = 4e-5; E.Rs_eq = 4e-3;
     E.c_squared = 3e4; E.over_c_squared = 1.0 / 3e4;

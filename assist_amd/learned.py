@@ -29,7 +29,7 @@ import os
 import numpy as np
 
 from ._libassist import clibassist
-from .batch import BatchConfig, BatchStats, DeviceArray, _check, _opts_ptr
+from .batch import BatchConfig, BatchExtra, BatchStats, DeviceArray, _check, _opts_ptr
 
 N_IN, N_HIDDEN, N_OUT = 16, 128, 32
 TRACE_WORDS = 34
@@ -54,15 +54,6 @@ clibassist.assist_mlp_adam_step.argtypes = [C.c_void_p, C.c_float, C.c_float, C.
 clibassist.assist_step_features.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
 clibassist.assist_batch_predict_start.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                                   C.c_void_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
-
-
-class BatchExtra(C.Structure):
-    """ctypes mirror of ``struct assist_batch_extra`` (every field, so that the library never
-    reads past the end of a Python-built struct; the dense-output pointers stay NULL here)."""
-    _fields_ = [("dt0", C.c_void_p), ("bw", C.c_void_p), ("trace", C.c_void_p), ("trace_count", C.c_void_p),
-                ("trace_capacity", C.c_int), ("trace_every", C.c_int), ("trace_skip", C.c_int),
-                ("reserved_", C.c_int), ("out_offset", C.c_void_p), ("out_t", C.c_void_p),
-                ("out_states", C.c_void_p)]
 
 
 def _p(d):
@@ -159,7 +150,8 @@ def integrate_traced(ephem, t0, t1, states, params=None, config: BatchConfig | N
     d_evals = DeviceArray(np.zeros(n, np.int64))
     d_trace = DeviceArray(np.zeros((max(capacity, 1), TRACE_WORDS)))
     d_count = DeviceArray(np.zeros(1, np.int32))
-    ex = BatchExtra(None, None, d_trace.ptr, d_count.ptr, int(capacity), int(every), int(skip), 0)
+    ex = BatchExtra.new(trace=d_trace.ptr, trace_count=d_count.ptr, trace_capacity=int(capacity),
+                        trace_every=int(every), trace_skip=int(skip))
     stats = BatchStats()
     cfg = config or BatchConfig.default()
     _check(clibassist.assist_batch_integrate_device_ex(
@@ -249,7 +241,7 @@ def predict_start(ephem, model: StepModel, n, n_var, d_t0, d_t1, d_states, min_d
 def integrate_device_learned(ephem, n, n_var, d_t0, d_t1, d_states, d_params, d_status, d_steps, d_evals, d_dt0,
                              d_bw=None, extras=None, config: BatchConfig | None = None) -> BatchStats:
     """IAS15 on device buffers from the learned start (d_dt0, d_bw from predict_start)."""
-    ex = BatchExtra(d_dt0.ptr, d_bw.ptr if d_bw is not None else None, None, None, 0, 1, 0, 0)
+    ex = BatchExtra.new(dt0=d_dt0.ptr, bw=d_bw.ptr if d_bw is not None else None)
     stats = BatchStats()
     cfg = config or BatchConfig.default()
     _check(clibassist.assist_batch_integrate_device_ex(

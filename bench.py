@@ -115,6 +115,40 @@ def _cpu_model():
     return ""
 
 
+def _cgroup_cpus():
+    """CPUs granted by the cgroup v2 (or v1) CPU quota, or None when unlimited or unreadable."""
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            return float(quota) / float(period)
+    except (OSError, ValueError):
+        pass
+    try:
+        with open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us") as f:
+            quota = float(f.read())
+        with open("/sys/fs/cgroup/cpu/cpu.cfs_period_us") as f:
+            period = float(f.read())
+        if quota > 0:
+            return quota / period
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def host_cpus():
+    """The host cores this process may run on: its CPU affinity set, limited by the cgroup CPU
+    quota when one is set (threads beyond the quota only time-share it).  Returns the thread count
+    to use and the facts it came from."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = _cgroup_cpus()
+    threads = affinity
+    if quota is not None:
+        threads = max(1, min(threads, int(quota + 0.5)))
+    return threads, {"nproc": os.cpu_count(), "affinity": affinity,
+                     "cgroup_cpu_quota": quota, "cpu_model": _cpu_model()}
+
+
 # Initial state of the reference's benchmark particle (reference unit_tests/benchmark/problem.c:72-76);
 # particle i is offset by i / 1e-10 au in x, as there.
 REFERENCE_BENCHMARK_STATE = [-2.724183384883979E+00, -3.523994546329214E-02, 9.036596202793466E-02,
@@ -146,7 +180,7 @@ def cpu_baseline(paths, cfg, t0, t1, states, params, sample_s=3.0, samples=5):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle
     o = Oracle(*paths)
-    cores = min(os.cpu_count() or 1, 16)
+    cores, host = host_cpus()
     prm = (lambda sl: None) if params is None else (lambda sl: params[sl])
     ocfg = o.cfg(min_dt=cfg["min_dt"])
     probe = 16
@@ -168,11 +202,13 @@ def cpu_baseline(paths, cfg, t0, t1, states, params, sample_s=3.0, samples=5):
     ts = time.perf_counter()
     o.integrate(t0[s1], t1[s1], states[s1], prm(s1), nthreads=1, cfg=ocfg)
     rate1 = float(np.sum(np.abs(t1[s1] - t0[s1]))) / (time.perf_counter() - ts)
-    cpu = _cpu_model()
     return dict(value=float(np.mean(rates)), std=float(np.std(rates, ddof=1)), samples=samples,
-                unit="particle-days/s", cores=cores, kind="port",
-                sample="first %d systems of the workload (%.0f particle-days) on %d threads, %d runs, %s"
-                       % (m, pdays, cores, samples, cpu),
+                unit="particle-days/s", cores=cores, kind="port", host=host,
+                sample="first %d systems of the workload (%.0f particle-days) on %d threads (the process's "
+                       "affinity set of %d CPUs%s), %d runs, %s"
+                       % (m, pdays, cores, host["affinity"],
+                          "" if host["cgroup_cpu_quota"] is None else ", cgroup quota %.1f CPUs" % host["cgroup_cpu_quota"],
+                          samples, host["cpu_model"]),
                 single_thread={"value": rate1, "sample": "first %d systems on 1 thread" % m1},
                 reference_benchmark_shape=cpu_reference_shape(o)), r, m
 
@@ -348,6 +384,9 @@ def main():
         "speedup_vs_plain_fp64": value / plain_value,
         "roofline": {"bound": "fp64-valu", "achieved": ach64 + ach32, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": frac, "traffic": traffic,
+                     "traffic_source": None if traffic is None else
+                     "committed counter profile profiles/ias15_kernel_traffic.md (rocprofv3 --pmc FETCH_SIZE, "
+                     "WRITE_SIZE of this workload and form, scaled by a copy kernel), not measured in this run",
                      "kernel": "ias15_kernel", "kernel_ms": kms, "flops_fp64_per_launch": f64,
                      "flops_fp32_per_launch": f32, "achieved_fp64": ach64, "achieved_fp32": ach32,
                      "hbm_gbs": None if traffic is None else traffic / (kms * 1e-3) / 1e9},

@@ -237,11 +237,27 @@ enum ASSIST_SYSTEM_STATUS {
     ASSIST_SYSTEM_NONFINITE = -1,
     ASSIST_SYSTEM_MAX_STEPS = -2,
     ASSIST_SYSTEM_DT_UNDERFLOW = -3,
+    ASSIST_SYSTEM_PAUSED = -4,       /* stopped at its pause time (assist_batch_extra.t_pause) */
 };
+
+/* Snapshot of one system's IAS15 state at a step boundary, in doubles (device memory):
+ *   header[ASSIST_SNAPSHOT_HEADER]: t, next trial step, last accepted step (signed; 0 before the
+ *     first step), steps, force evaluations, rejected attempts, status, lanes per system of the
+ *     kernel that took it;
+ *   then per vector (real particle first) ASSIST_SNAPSHOT_VECTOR doubles: x[3], v[3], the Kahan
+ *     compensations csx[3], csv[3], the acceleration a0[3] at the start of the last accepted step,
+ *     that step's b[7][3] and the correction e[7][3] of its prediction (k-major: b[3k + c]).
+ * A checkpoint holds everything IAS15 carries from one step to the next, so that an integration
+ * resumed from it takes the same steps as one that was never interrupted. */
+#define ASSIST_SNAPSHOT_HEADER 8
+#define ASSIST_SNAPSHOT_VECTOR 57
+#define ASSIST_SNAPSHOT_WORDS(nvec) (ASSIST_SNAPSHOT_HEADER + ASSIST_SNAPSHOT_VECTOR * (nvec))
 
 enum ASSIST_PRECISION { ASSIST_PRECISION_FP64 = 0, ASSIST_PRECISION_MIXED = 1 };
 
 struct assist_batch_config {
+    size_t size;          /* sizeof(struct assist_batch_config), set by assist_batch_config_default;
+                             the entry points reject any other value (ASSIST_BATCH_ERROR_ARG) */
     double epsilon;       /* IAS15 epsilon (default 1e-9); 0 = fixed step */
     double min_dt;        /* default 0 */
     double dt_initial;    /* first trial step in days (default 0.001) */
@@ -268,6 +284,9 @@ struct assist_batch_stats {
 };
 
 void assist_batch_config_default(struct assist_batch_config* cfg);
+struct assist_batch_extra;
+/* Zeroes every field of `ex`, sets its size and trace_every = 1. */
+void assist_batch_extra_init(struct assist_batch_extra* ex);
 const char* assist_batch_error_string(int code);
 /* Number of visible GPUs (0 when none / runtime unavailable). */
 int assist_gpu_device_count(void);
@@ -309,8 +328,10 @@ int assist_batch_integrate_device(const struct assist_ephem* ephem, const struct
                                   long long* d_steps, long long* d_evals, void* stream,
                                   struct assist_batch_stats* stats);
 
-/* Optional inputs and outputs of assist_batch_integrate_device_ex (device pointers). */
+/* Optional inputs and outputs of assist_batch_integrate_device_ex (device pointers).  Initialise
+ * with assist_batch_extra_init; the entry points reject a struct whose size field differs. */
 struct assist_batch_extra {
+    size_t size;              /* sizeof(struct assist_batch_extra) */
     const double* dt0;        /* [n] first trial step per system (days; sign taken from t1 - t0), or NULL */
     const double* bw;         /* [n][21] warm start of the real particle's first step, b[k][c] =
                                  bw[3k + c] |a0| dt^(k+1), or NULL */
@@ -329,6 +350,25 @@ struct assist_batch_extra {
     const int* out_offset;
     const double* out_t;
     double* out_states;
+    /* Checkpoint and resume (all NULL for none), ASSIST_SNAPSHOT_WORDS(1 + n_var) doubles per
+     * system.  snapshot_in: each system resumes from its snapshot; t0 and dt0 are not read, t1
+     * is the end as before, and the lanes per system recorded in the snapshot are used unless
+     * the config sets them.  Not with dense output (ASSIST_BATCH_ERROR_UNSUPPORTED).
+     * snapshot_out: each system's snapshot when it stops (finished, paused or failed).
+     * t_pause: system i stops with status ASSIST_SYSTEM_PAUSED at the first step boundary at or
+     * past t_pause[i] in its direction (unless that boundary is t1). */
+    const double* snapshot_in;
+    double* snapshot_out;
+    const double* t_pause;
+    /* Step archive (NULL for none; the counterpart of REBOUND's SimulationArchive saved every
+     * step, which reference src/assist.c:599-633 reads): system i writes its snapshot at its
+     * start and after every accepted step to archive[i][k][ASSIST_SNAPSHOT_WORDS(1 + n_var)],
+     * k < archive_capacity, and the number of snapshots it took to archive_count[i] (more than
+     * the capacity when it ran out; the later ones are not written). */
+    double* archive;
+    int* archive_count;
+    int archive_capacity;
+    int reserved2_;
 };
 
 /* Same as assist_batch_integrate_device, with the learned start and step traces of `ex`. */
@@ -346,6 +386,20 @@ int assist_batch_integrate_dense(const struct assist_ephem* ephem, const struct 
                                  const struct assist_batch_config* cfg, int n, int n_var, const double* t0,
                                  double* states, const double* params, const int* out_offset, const double* out_t,
                                  double* out_states, int* status, long long* steps, struct assist_batch_stats* stats);
+
+/* States at arbitrary times from step archives (struct assist_batch_extra.archive): query q
+ * asks for system q_sys[q] at q_t[q]; out_states[q][1 + n_var][6].  The state is the IAS15
+ * polynomial of the archived step that contains the time, from the earlier snapshot's state and
+ * the later one's a0, b and step (reference assist_interpolate_simulation, src/assist.c:682-752,
+ * with the blob lookup of assist_create_interpolated_simulation, :599-633); a time equal to a
+ * snapshot's returns its state.  Unlike the reference, the first step interpolates too (every
+ * snapshot holds its step's a0).  Per-query status: 0, or ASSIST_ERROR_COVERAGE outside the
+ * archived steps (out is zero then).  Device buffers and a stream; host buffers below. */
+int assist_batch_archive_interpolate_device(int n, int n_var, const double* d_archive, int capacity,
+                                            const int* d_count, int nq, const int* d_q_sys, const double* d_q_t,
+                                            double* d_out_states, int* d_status, void* stream);
+int assist_batch_archive_interpolate(int n, int n_var, const double* archive, int capacity, const int* count, int nq,
+                                     const int* q_sys, const double* q_t, double* out_states, int* status);
 
 /* The reference's dense-output call (reference src/assist.c:642-680): brings the simulation to
  * time t by integrating every real particle (with its variational particles) as its own system

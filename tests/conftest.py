@@ -22,6 +22,16 @@ def ephem_paths():
 
 
 @pytest.fixture(scope="session")
+def short_asteroid_paths(tmp_path_factory):
+    """The seed-0 planets with an asteroid file covering only JD 2451545 +/- 3000 days."""
+    from assist_amd import synthetic
+    d = tmp_path_factory.mktemp("short_asteroids")
+    pp, ap = str(d / "planets.bsp"), str(d / "asteroids.bsp")
+    synthetic.write_synthetic_ephemeris(pp, ap, 0, asteroid_coverage_days=3000.0)
+    return pp, ap
+
+
+@pytest.fixture(scope="session")
 def model():
     from assist_amd import synthetic
     return synthetic.SyntheticModel(0)

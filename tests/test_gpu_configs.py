@@ -1,0 +1,161 @@
+"""Parity of the benchmark configurations 3 and 5 on their own kernels and sizes.
+
+Configuration 5 (1 048 576 real-only systems, 70/30 main-belt/NEO, 365 days, mixed precision with
+the learned start) runs on one lane per system, the kernel the autotuner chooses for it
+(ias15_kernel<1, true, 1>): against the FP64 oracle on samples, and in full through
+size-independent properties.  Configuration 3 (65 536 systems, log-uniform spans of 30-10 000
+days, half backward, 10% with a Marsden model, min_dt 0.001) against the oracle on the first 512
+systems of the workload itself (the longest spans included), and in full through properties.
+
+The 1 m bars are stated for orbits without close approaches.  An orbit is exempt only when it is
+outside the bar *and* its measured trajectory comes within max(0.1 au, 3 Hill radii) of a planet
+(tests/encounters.py); every exemption is printed with its planet and distance ratio.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import AU_M
+from encounters import bar_with_measured_exemptions
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(len(os.sched_getaffinity(0)), 16)
+
+
+def _workload(name, model, n=None):
+    from bench import CONFIGS, make_workload
+    cfg = dict(CONFIGS[name])
+    if n is not None:
+        cfg["n"] = n
+    return make_workload(cfg, model)
+
+
+def _net():
+    import assist_amd.learned as L
+    return L.StepModel.load(L.default_model_path(), max_batch=65536)
+
+
+def _run(ephem, t0, t1, states, params, cfg, net=None):
+    """Device-resident integration (optionally from the learned start); host results."""
+    from assist_amd import batch
+    import assist_amd.learned as L
+    n, nv = states.shape[:2]
+    d_t0, d_t1, d_st = batch.DeviceArray(t0), batch.DeviceArray(t1), batch.DeviceArray(states)
+    d_pp = batch.DeviceArray(params) if params is not None else None
+    d_status = batch.DeviceArray(np.zeros(n, np.int32))
+    d_steps = batch.DeviceArray(np.zeros(n, np.int64))
+    d_evals = batch.DeviceArray(np.zeros(n, np.int64))
+    if net is None:
+        stats = batch.integrate_device(ephem, n, nv - 1, d_t0, d_t1, d_st, d_pp, d_status, d_steps, d_evals, config=cfg)
+    else:
+        d_dt0, d_bw = L.predict_start(ephem, net, n, nv - 1, d_t0, d_t1, batch.DeviceArray(states),
+                                      min_dt=cfg.min_dt)
+        stats = L.integrate_device_learned(ephem, n, nv - 1, d_t0, d_t1, d_st, d_pp, d_status, d_steps, d_evals,
+                                           d_dt0, d_bw, config=cfg)
+    return dict(states=d_st.download(), status=d_status.download(), steps=d_steps.download(),
+                evals=d_evals.download(), stats=stats)
+
+
+def test_configuration_5_one_lane_mixed_learned_against_oracle(gpu, ephem, oracle, model):
+    """256 systems of configuration 5 (its own seed and mix), one lane each, mixed precision,
+    learned start, 365 days, against the FP64 oracle."""
+    from assist_amd import batch
+    t0, t1, states, params = _workload("c5", model, n=256)
+    cfg = batch.BatchConfig.default(lanes_per_system=1, precision=batch.PRECISION_MIXED, min_dt=0.001,
+                                    max_steps=200000)
+    r = _run(ephem, t0, t1, states, params, cfg, _net())
+    assert r["stats"].lanes_per_system == 1
+    assert (r["status"] == 0).all()
+    o = oracle.integrate(t0, t1, states, params, cfg=oracle.cfg(min_dt=0.001), nthreads=NTHREADS)
+    assert (o["status"] == 0).all()
+    d = np.linalg.norm(r["states"][:, 0, :3] - o["states"][:, 0, :3], axis=-1) * AU_M
+    print("evaluations: one-lane mixed+learned %d, oracle %d" % (r["evals"].sum(), o["evals"].sum()))
+    bar_with_measured_exemptions(ephem, "configuration 5 sample, one lane, mixed+learned vs oracle", d, t0, t1,
+                                  states, params)
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_one_lane_mixed_main_belt_10yr_against_oracle(gpu, ephem, oracle, model, learned):
+    """64 main-belt systems, one lane each, mixed precision, 3650 days: within 1 m of the FP64
+    oracle (no exemptions)."""
+    from assist_amd import batch, synthetic
+    rng = np.random.default_rng(57)
+    n = 64
+    states = np.zeros((n, 1, 6))
+    states[:, 0] = synthetic.main_belt(model, n, rng)
+    t0, t1 = np.zeros(n), np.full(n, 3650.0)
+    cfg = batch.BatchConfig.default(lanes_per_system=1, precision=batch.PRECISION_MIXED)
+    r = _run(ephem, t0, t1, states, None, cfg, _net() if learned else None)
+    assert r["stats"].lanes_per_system == 1 and (r["status"] == 0).all()
+    o = oracle.integrate(t0, t1, states, nthreads=NTHREADS)
+    d = np.linalg.norm(r["states"][:, 0, :3] - o["states"][:, 0, :3], axis=-1) * AU_M
+    print("one lane, mixed%s, 10 yr main belt: max |dr| %.3e m" % ("+learned" if learned else "", d.max()))
+    assert d.max() <= 1.0
+
+
+def test_configuration_5_full_size_properties(gpu, ephem, model):
+    """Configuration 5 in full on one lane per system: every status 0; two mixed+learned launches
+    bit-identical; mixed+learned within 1 m of the plain FP64 form, exemptions measured."""
+    from assist_amd import batch
+    t0, t1, states, params = _workload("c5", model)
+    n = len(t0)
+    assert n == 1048576 and states.shape[1] == 1
+    net = _net()
+    mixed = batch.BatchConfig.default(lanes_per_system=1, precision=batch.PRECISION_MIXED, min_dt=0.001,
+                                      max_steps=200000)
+    a = _run(ephem, t0, t1, states, params, mixed, net)
+    b = _run(ephem, t0, t1, states, params, mixed, net)
+    assert a["stats"].lanes_per_system == 1
+    assert (a["status"] == 0).all()
+    for k in ("states", "status", "steps", "evals"):
+        assert np.array_equal(a[k], b[k]), k
+    f = _run(ephem, t0, t1, states, params,
+             batch.BatchConfig.default(lanes_per_system=1, min_dt=0.001, max_steps=200000))
+    assert (f["status"] == 0).all()
+    d = np.linalg.norm(a["states"][:, 0, :3] - f["states"][:, 0, :3], axis=-1) * AU_M
+    print("configuration 5 in full: evaluations mixed+learned %d, FP64 %d (%.3fx)"
+          % (a["evals"].sum(), f["evals"].sum(), f["evals"].sum() / a["evals"].sum()))
+    bar_with_measured_exemptions(ephem, "configuration 5 in full, mixed+learned vs FP64", d, t0, t1, states, params)
+
+
+def test_configuration_3_sample_against_oracle(gpu, ephem, oracle, model):
+    """The first 512 systems of configuration 3 (spans log-uniform in 30-10 000 days, half
+    backward, 10% Marsden, min_dt 0.001) against the oracle, each held to 1 m unless a close
+    approach is measured on its own trajectory."""
+    from assist_amd import batch
+    t0, t1, states, params = _workload("c3", model)
+    sl = slice(0, 512)
+    t0, t1, states, params = t0[sl], t1[sl], states[sl], params[sl]
+    cfg = batch.BatchConfig.default(min_dt=0.001, max_steps=200000)
+    r = _run(ephem, t0, t1, states, params, cfg)
+    assert (r["status"] == 0).all()
+    o = oracle.integrate(t0, t1, states, params, cfg=oracle.cfg(min_dt=0.001), nthreads=NTHREADS)
+    assert (o["status"] == 0).all()
+    d = np.linalg.norm(r["states"][:, 0, :3] - o["states"][:, 0, :3], axis=-1) * AU_M
+    span = np.abs(t1 - t0)
+    ng = np.any(params[:, 0] != 0, axis=-1)
+    print("configuration 3 sample: spans %.0f-%.0f d (%d over 5 000 d, %d backward), %d with a Marsden model"
+          % (span.min(), span.max(), (span > 5000).sum(), (t1 < t0).sum(), ng.sum()))
+    bar_with_measured_exemptions(ephem, "configuration 3 sample vs oracle", d, t0, t1, states, params)
+
+
+def test_configuration_3_full_size_properties(gpu, ephem, model):
+    """Configuration 3 in full: every status 0; two launches bit-identical (the work queue hands
+    systems out in a different order each time); integrating the end states back to t0 returns
+    them to their start within 1 m unless a close approach is measured."""
+    from assist_amd import batch
+    t0, t1, states, params = _workload("c3", model)
+    n = len(t0)
+    assert n == 65536
+    cfg = batch.BatchConfig.default(min_dt=0.001, max_steps=200000)
+    a = _run(ephem, t0, t1, states, params, cfg)
+    b = _run(ephem, t0, t1, states, params, cfg)
+    assert (a["status"] == 0).all()
+    for k in ("states", "status", "steps", "evals"):
+        assert np.array_equal(a[k], b[k]), k
+    back = _run(ephem, t1, t0, a["states"], params, cfg)
+    assert (back["status"] == 0).all()
+    d = np.linalg.norm(back["states"][:, 0, :3] - states[:, 0, :3], axis=-1) * AU_M
+    bar_with_measured_exemptions(ephem, "configuration 3 in full, out and back", d, t0, t1, states, params)

@@ -47,3 +47,37 @@ def test_batched_query_errors(gpu, ephem):
     out, st = batch.get_particles(ephem, [0, 5, -1, 27, 12], [10000.5, -10001.0, 0.0, 0.0, 0.0])
     assert list(st) == [5, 5, 4, 3, 0]
     assert (out[:4] == 0).all() and out[4, 0] > 0
+
+
+def test_coverage_windows_per_file(gpu, short_asteroid_paths):
+    """With an asteroid file shorter than the planets file, the GPU force model and ephemeris
+    queries report coverage as the oracle (the reference's rule) does: planet queries and force
+    models without a direct term work where only the planets cover."""
+    import assist_amd
+    from assist_amd import batch
+    from oracle_lib import Oracle
+    from test_library import COVERAGE_CASES
+    pp, ap = short_asteroid_paths
+    e = assist_amd.Ephem(pp, ap)
+    o = Oracle(pp, ap)
+    out, st = batch.get_particles(e, [5, 12, 5, 12, 0], [5000.0, 5000.0, 2000.0, 2000.0, -9000.0])
+    np.testing.assert_array_equal(st, [0, 5, 0, 0, 0])
+    st0 = np.zeros((2, 2, 6))
+    st0[:, 0] = [2.5, 0.1, 0.05, -0.001, 0.01, 0.002]
+    st0[:, 1, 0] = 1.0
+    t = np.array([2000.0, 5000.0])
+    for forces, expect in COVERAGE_CASES:
+        for prec in (batch.PRECISION_FP64, batch.PRECISION_MIXED):
+            acc, s = batch.forces(e, t, st0, extras={"forces": forces}, precision=prec)
+            ref, so = o.forces(t, st0, opts=o.opts(forces=forces))
+            np.testing.assert_array_equal(s, [0, expect])
+            np.testing.assert_array_equal(s, so)
+            ok = s == 0
+            if forces and prec == batch.PRECISION_FP64:
+                err = np.abs(acc[ok] - ref[ok]).max() / np.abs(ref[ok]).max()
+                assert err <= 1e-12, (forces, err)
+    # an integration the planets cover but the asteroids do not: fine without the direct term
+    r = batch.integrate(e, 4000.0, 4100.0, st0[:1, :1], extras={"forces": 0x10 | 0x40})
+    assert r.status[0] == 0
+    r = batch.integrate(e, 4000.0, 4100.0, st0[:1, :1])
+    assert r.status[0] == 5

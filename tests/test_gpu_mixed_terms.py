@@ -1,0 +1,93 @@
+"""The mixed-precision form term by term against the FP64 oracle.
+
+In the mixed form the Chebyshev series, the Sun, planet and Earth-Moon point masses, Earth J2,
+solar J2 and the IAS15 sums stay in FP64; the 16 asteroid point masses, Earth J3/J4, GR (EIH,
+simple, potential), the Marsden term and every variational partial derivative are evaluated in
+FP32 from FP64 differences (an asteroid closer than 0.2 au falls back to FP64).  Each term is
+evaluated alone (J3 and J4 from ephemerides whose other zonal coefficients are zero) on 1 024
+random systems x (1 + 6 variational) with Marsden models, and its error is printed as the max and
+the 99th percentile of
+
+  * |a_mixed - a_oracle| / |a_oracle| for the term's own real-particle acceleration,
+  * the same over the variational vectors,
+  * |a_mixed - a_oracle| / |a_full| -- the term's error against the whole real acceleration,
+    which is what enters the integration.
+
+The bounds asserted are the ones stated for the mixed form: FP64 terms 1e-12 of themselves, FP32
+terms 2e-6 of themselves and 1e-11 of the whole acceleration, variational partials 1e-5.
+"""
+import numpy as np
+import pytest
+
+from conftest import population, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP64_TERMS = {"SUN": 0x01, "PLANETS": 0x02, "EARTH_J2": 0x10, "SUN_HARMONICS": 0x20}
+FP32_TERMS = {"ASTEROIDS": 0x04, "EARTH_J3": 0x10, "EARTH_J4": 0x10, "NON_GRAVITATIONAL": 0x08, "GR_EIH": 0x40,
+              "GR_SIMPLE": 0x80, "GR_POTENTIAL": 0x100}
+
+
+@pytest.fixture(scope="module")
+def zonal(tmp_path_factory):
+    """(Ephem, Oracle) pairs whose planets file has one non-zero zonal coefficient."""
+    import assist_amd
+    from assist_amd import synthetic
+    from oracle_lib import Oracle
+    d = tmp_path_factory.mktemp("zonal_mixed")
+    out = {}
+    for n in (2, 3, 4):
+        c = {"J2E": 0.0, "J3E": 0.0, "J4E": 0.0}
+        c["J%dE" % n] = synthetic.CONSTANTS["J%dE" % n]
+        pp, ap = str(d / ("p%d.bsp" % n)), str(d / ("a%d.bsp" % n))
+        synthetic.write_synthetic_ephemeris(pp, ap, 0, constants=c)
+        out[n] = (assist_amd.Ephem(pp, ap), Oracle(pp, ap))
+    return out
+
+
+def _sample(model, n, seed):
+    rng = np.random.default_rng(seed)
+    states, params, _ = population(model, n, seed, n_var=6, ng_fraction=1.0)
+    t = rng.uniform(-9500.0, 9500.0, n)
+    states[:, 1:] = rng.normal(size=states[:, 1:].shape)
+    params[:, 6] = rng.normal(size=(n, 3)) * 1e-12
+    return t, states, params
+
+
+def _stats(x):
+    return float(np.max(x)), float(np.percentile(x, 99))
+
+
+def test_mixed_form_error_of_each_term(gpu, ephem, oracle, model, zonal):
+    from assist_amd import batch
+    t, states, params = _sample(model, 1024, 61)
+    full, st = oracle.forces(t, states, params, oracle.opts(forces=0x7F))
+    assert (st == 0).all()
+    amag = np.linalg.norm(full[:, 0], axis=-1)
+    rows = []
+    for name, forces in list(FP64_TERMS.items()) + list(FP32_TERMS.items()):
+        e, o = ephem, oracle
+        if name.startswith("EARTH_J"):
+            e, o = zonal[int(name[-1])]
+        g, gs = batch.forces(e, t, states, params, extras={"forces": forces}, precision=batch.PRECISION_MIXED)
+        r, rs = o.forces(t, states, params, o.opts(forces=forces))
+        assert (gs == 0).all() and (rs == 0).all()
+        nz = np.linalg.norm(r[:, 0], axis=-1) > 0
+        own = rel_err(g[nz, 0], r[nz, 0])
+        var = rel_err(g[:, 1:], r[:, 1:]).ravel()
+        var = var[np.isfinite(var)]
+        whole = np.linalg.norm(g[:, 0] - r[:, 0], axis=-1) / amag
+        rows.append((name, name in FP32_TERMS, _stats(own), _stats(var), _stats(whole)))
+    print("\nmixed form, each term alone vs the FP64 oracle (1024 systems x 7 vectors):")
+    print("%-18s %-5s %-22s %-22s %-22s" % ("term", "prec", "own real (max / p99)", "variational (max / p99)",
+                                             "of |a_full| (max / p99)"))
+    for name, fp32, own, var, whole in rows:
+        print("%-18s %-5s %.2e / %.2e      %.2e / %.2e      %.2e / %.2e"
+              % (name, "FP32" if fp32 else "FP64", own[0], own[1], var[0], var[1], whole[0], whole[1]))
+    for name, fp32, own, var, whole in rows:
+        if fp32:
+            assert own[0] <= 2e-6, name
+            assert whole[0] <= 1e-11, name
+        else:
+            assert own[0] <= 1e-12, name
+        assert var[0] <= 1e-5, name

@@ -152,7 +152,8 @@ def _c_layout(struct_name, fields):
     """(sizeof, {field: offsetof}) of a struct of include/assist.h, from gcc."""
     import json
     import subprocess
-    src = "#include <stdio.h>\n#include <stddef.h>\n#include \"assist.h\"\nint main(void){printf(\"{\\\"size\\\": %zu\", sizeof(struct " + struct_name + "));"
+    src = ("#include <stdio.h>\n#include <stddef.h>\n#include \"assist.h\"\n#include \"spk.h\"\n"
+           "int main(void){printf(\"{\\\"__sizeof__\\\": %zu\", sizeof(struct " + struct_name + "));")
     for f in fields:
         src += "printf(\", \\\"%s\\\": %%zu\", offsetof(struct %s, %s));" % (f, struct_name, f)
     src += "printf(\"}\\n\"); return 0;}\n"
@@ -161,7 +162,7 @@ def _c_layout(struct_name, fields):
         subprocess.run(["gcc", "-x", "c", "-I", os.path.join(ROOT, "include"), "-o", exe, "-"], input=src.encode(),
                        check=True)
         out = json.loads(subprocess.run([exe], check=True, capture_output=True).stdout)
-    return out.pop("size"), out
+    return out.pop("__sizeof__"), out
 
 
 def test_struct_mirrors_match_the_header():
@@ -169,15 +170,17 @@ def test_struct_mirrors_match_the_header():
     offsets and the C size (a short mirror makes the library read or write past the end of the
     caller's struct).  Mirror fields named with a leading underscore wrap the C field."""
     import assist_amd
-    from assist_amd.batch import BatchConfig, BatchStats
+    from assist_amd.batch import BatchConfig, BatchExtra, BatchStats
     from assist_amd.extras import Extras
-    from assist_amd.learned import BatchExtra
+    from assist_amd import spk
     from assist_amd.simulation import IntegratorIAS15, Particle, Simulation, VariationalConfiguration
     mirrors = ((BatchConfig, "assist_batch_config"), (BatchStats, "assist_batch_stats"),
                (BatchExtra, "assist_batch_extra"), (assist_amd.Ephem, "assist_ephem"), (Extras, "assist_extras"),
                (Particle, "reb_particle"), (Simulation, "reb_simulation"),
                (VariationalConfiguration, "reb_variational_configuration"),
-               (IntegratorIAS15, "reb_simulation_integrator_ias15"))
+               (IntegratorIAS15, "reb_simulation_integrator_ias15"),
+               (spk.SpkFile, "spk_s"), (spk.SpkTarget, "spk_target"), (spk.MassData, "mass_data"),
+               (spk.ConstantsAndMasses, "spk_constants_and_masses"), (spk.Mpos, "mpos_s"))
     for cls, name in mirrors:
         fields = [f[0] for f in cls._fields_]
         cnames = [f.lstrip("_") if f != "reserved_" else f for f in fields]
@@ -195,9 +198,51 @@ def test_batch_config_default_sets_every_field():
     c = BatchConfig()
     C.memset(C.addressof(c), 0x5A, C.sizeof(c))
     assist_amd.clibassist.assist_batch_config_default(C.byref(c))
+    assert c.size == C.sizeof(BatchConfig)
     assert (c.epsilon, c.min_dt, c.dt_initial, c.max_steps, c.precision) == (1e-9, 0.0, 0.001, 1000000, 0)
     assert (c.lanes_per_system, c.block_size, c.blocks_per_cu, c.waves_per_simd, c.use_graph,
             c.substep_cache_off) == (0, 0, 0, 0, 0, 0)
+
+
+def test_structs_of_another_size_are_rejected(ephem):
+    """A config or extra struct whose size field is not the library's (a caller built against
+    another layout) is rejected with ASSIST_BATCH_ERROR_ARG before anything is read past it or
+    any device is touched."""
+    import assist_amd
+    from assist_amd.batch import BatchConfig, BatchExtra
+    lib = assist_amd.clibassist
+    e = BatchExtra.new()
+    assert e.size == C.sizeof(BatchExtra) and e.trace_every == 1
+    assert all(getattr(e, f) in (None, 0) for f, _ in BatchExtra._fields_ if f not in ("size", "trace_every"))
+    t = np.zeros(1)
+    st = np.zeros((1, 1, 6))
+    status = np.zeros(1, dtype=np.int32)
+    dp = C.POINTER(C.c_double)
+    for bad in (0, C.sizeof(BatchConfig) - 8, C.sizeof(BatchConfig) + 8):
+        c = BatchConfig.default()
+        c.size = bad
+        rc = lib.assist_batch_integrate(C.byref(ephem), None, C.byref(c), 1, 0, t.ctypes.data_as(dp),
+                                        t.ctypes.data_as(dp), st.ctypes.data_as(dp), None,
+                                        status.ctypes.data_as(C.POINTER(C.c_int)), None, None)
+        assert rc == 100
+        rc = lib.assist_batch_integrate_device_ex(C.byref(ephem), None, C.byref(c), 1, 0, C.c_void_p(8), C.c_void_p(8),
+                                                  C.c_void_p(8), None, C.c_void_p(8), None, None, None, None, None)
+        assert rc == 100
+        x = BatchExtra.new()
+        x.size = bad
+        rc = lib.assist_batch_integrate_device_ex(C.byref(ephem), None, None, 1, 0, C.c_void_p(8), C.c_void_p(8),
+                                                  C.c_void_p(8), None, C.c_void_p(8), None, None, None, None,
+                                                  C.byref(x))
+        assert rc == 100
+    # an archive needs its counts and a capacity; a resume cannot take dense output
+    x = BatchExtra.new(archive=8, archive_capacity=0, archive_count=8)
+    assert lib.assist_batch_integrate_device_ex(C.byref(ephem), None, None, 1, 0, C.c_void_p(8), C.c_void_p(8),
+                                                C.c_void_p(8), None, C.c_void_p(8), None, None, None, None,
+                                                C.byref(x)) == 100
+    x = BatchExtra.new(snapshot_in=8, out_t=8, out_offset=8, out_states=8)
+    assert lib.assist_batch_integrate_device_ex(C.byref(ephem), None, None, 1, 0, C.c_void_p(8), None,
+                                                C.c_void_p(8), None, C.c_void_p(8), None, None, None, None,
+                                                C.byref(x)) == 102
 
 
 def test_gpu_entry_points_fail_without_gpu(ephem):
@@ -307,3 +352,36 @@ def test_cpu_baseline_protocol(ephem_paths):
     shape = cb["reference_benchmark_shape"]
     assert shape["samples"] == 5 and shape["steps"] > 0 and shape["evals"] >= 8 * shape["steps"]
     assert r["states"].shape == (m, 7, 6) and (r["status"] == 0).all()
+
+
+# force flags (enum ASSIST_FORCES) of the coverage cases below, and the status the reference's
+# rule gives at a time the planets cover and the asteroids do not: the direct term reports
+# coverage errors (its variational loop fetches the first asteroid whatever the flags, reference
+# src/forces.c:351-367), the other terms do not check them
+COVERAGE_CASES = [(0x7F, 5), (0x03, 5), (0x04, 5), (0x10 | 0x40, 0), (0x20 | 0x08, 0), (0x00, 0)]
+
+
+def test_coverage_windows_per_file(short_asteroid_paths, oracle):
+    """With an asteroid file shorter than the planets file: time bounds are the intersection (as
+    reference assist_ephem_time_bounds), planet queries succeed where only the planets cover,
+    asteroid queries fail there, and the oracle's force model follows the reference's rule."""
+    import assist_amd
+    from oracle_lib import Oracle
+    pp, ap = short_asteroid_paths
+    e = assist_amd.Ephem(pp, ap)
+    o = Oracle(pp, ap)
+    assert e.time_bounds() == (-3000.0, 3000.0)
+    for t in (5000.0, -9000.0):
+        p = e.get_particle(5, t)
+        f, ref = o.all_ephem(5, t)
+        assert f == 0 and (p.x, p.y, p.z) == tuple(ref[1:4])
+        with pytest.raises(RuntimeError):
+            e.get_particle(12, t)
+        assert o.all_ephem(12, t)[0] == 5
+    st = np.zeros((1, 2, 6))
+    st[0, 0] = [2.5, 0.1, 0.05, -0.001, 0.01, 0.002]
+    st[0, 1, 0] = 1.0
+    for forces, expect in COVERAGE_CASES:
+        _, s_in = o.forces(2000.0, st, opts=o.opts(forces=forces))
+        _, s_out = o.forces(5000.0, st, opts=o.opts(forces=forces))
+        assert (s_in[0], s_out[0]) == (0, expect), forces

@@ -142,3 +142,26 @@ def test_coverage_stops_the_integration(oracle, model):
     r = oracle.integrate(9990.0, 10020.0, x)
     assert r["status"][0] == 5
     assert 9990.0 <= r["t"][0] <= 10000.0
+
+
+def test_failure_codes(oracle, model):
+    """The oracle's per-system codes for host-forced failures (the GPU tests hold the kernel to
+    them): -1 non-finite state, -2 max_steps, -3 a zero or NaN first step; the state left is the
+    last accepted step's."""
+    x = synthetic.config1_particle(model)
+    st = np.zeros((3, 2, 6))
+    st[:, 0] = x
+    st[:, 1, 0] = 1.0
+    st[1, 0, 1] = np.nan
+    st[2, 1, 5] = np.inf
+    r = oracle.integrate(0.0, 30.0, st)
+    assert list(r["status"]) == [0, -1, -1]
+    r = oracle.integrate(0.0, 200.0, st[:1], cfg=oracle.cfg(max_steps=3))
+    assert r["status"][0] == -2 and r["steps"][0] == 3 and 0.0 < r["t"][0] < 200.0
+    ref = oracle.integrate(0.0, r["t"][0], st[:1], cfg=oracle.cfg())
+    assert ref["steps"][0] == 3 and ref["status"][0] == 0
+    np.testing.assert_array_equal(r["states"], ref["states"])
+    for dt in (0.0, np.nan):
+        r = oracle.integrate(0.0, 30.0, st[:1], cfg=oracle.cfg(dt=dt))
+        assert r["status"][0] == -3 and r["steps"][0] == 0
+        np.testing.assert_array_equal(r["states"], st[:1])
