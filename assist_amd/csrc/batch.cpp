@@ -275,6 +275,16 @@ int make_force_opts(const assist_extras* x, ForceOpts* fo) {
     return ASSIST_SUCCESS;
 }
 
+// The kernel's predictor-corrector tolerance for a configuration (IntegratorOpts::pc_tol): the
+// mixed form's FP32 terms carry round-off far above FP64's, so by default it stops once the real
+// particle's b6 changes by less than epsilon / 100 of its acceleration -- a hundredth of the
+// step's own truncation-error scale -- rather than at FP64 round-off over every vector.
+double corrector_tolerance(const assist_batch_config& cfg) {
+    if (cfg.corrector_tolerance > 0) return cfg.corrector_tolerance;
+    if (cfg.corrector_tolerance < 0 || cfg.precision != ASSIST_PRECISION_MIXED || !(cfg.epsilon > 0)) return 0.0;
+    return fmax(1e-16, 1e-2 * cfg.epsilon);
+}
+
 int lanes_for(int nvec, int requested) {
     int g = 1;
     while (g < nvec) g <<= 1;
@@ -801,6 +811,7 @@ static int integrate_device_locked(const struct assist_ephem* ephem, const struc
     a.io.min_dt = cfg.min_dt;
     a.io.dt_initial = cfg.dt_initial;
     a.io.max_steps = cfg.max_steps;
+    a.io.pc_tol = corrector_tolerance(cfg);
     a.nsys = n;
     a.nvec = nv;
     a.t0 = d_t0;

@@ -82,6 +82,9 @@ struct IntegratorOpts {
     double min_dt;
     double dt_initial;
     long long max_steps;
+    // predictor-corrector stop: 0 = IAS15's own test, max|db6| / max|a| over every vector of the
+    // system below 1e-16; > 0 = the real particle's max|db6| / max|a| below pc_tol
+    double pc_tol;
 };
 
 enum SystemStatus : int {

@@ -273,6 +273,13 @@ struct assist_batch_config {
     int substep_cache_off; /* 0 (default): 8-lane groups keep the ephemeris positions of each
                              substep for the later predictor-corrector iterations of the step
                              (results are the same bit for bit); 1: evaluate them every time */
+    int reserved_;
+    double corrector_tolerance; /* when the predictor-corrector iterations stop.  0 (default): by
+                             precision -- FP64 takes IAS15's own test (every vector's
+                             max|db6| / max|a| below 1e-16, as REBOUND does); the mixed form
+                             stops when the real particle's max|db6| / max|a| is below
+                             epsilon / 100 (1e-16 when epsilon is 0).  > 0: that tolerance on the
+                             real particle, in either precision.  < 0: IAS15's own test. */
 };
 
 struct assist_batch_stats {

@@ -7,7 +7,9 @@
  *  - the acceleration expanded in the Newton basis on the 7 Gauss-Radau
  *    spacings (g values) and in powers of h (b values),
  *  - predictor-corrector iterations until max|delta b6| / max|a| < 1e-16
- *    (at most 12; stop early when the error stops decreasing after 3),
+ *    (at most 12; stop early when the error stops decreasing after 3); as an
+ *    option of the product (cfg->pc_tol > 0), until the real particles' own
+ *    max|delta b6| / max|a| is below pc_tol,
  *  - the global step-size criterion dt' = dt (eps / (max|b6| / max|a|))^(1/7)
  *    over the real particles (ASSIST selects this "legacy global" mode,
  *    reference src/assist.c:446), rejection when dt' < dt/4, growth <= 4x,
@@ -113,6 +115,7 @@ void orc_ias15_cfg_default(orc_ias15_cfg *c) {
     c->dt = 0.001;
     c->exact_finish = 1;
     c->max_steps = 100000000L;
+    c->pc_tol = 0.0;
 }
 
 static inline void add_cs(double *p, double *cs, double in) {
@@ -234,8 +237,10 @@ int orc_integrate(const orc_ephem *eph, const orc_opts *o, orc_ias15_cfg *cfg, o
             }
             double pc_err = 1e300, pc_err_last = 2.0;
             int it = 0;
+            const double pc_tol = cfg->pc_tol > 0 ? cfg->pc_tol : 1e-16;
+            const int pc_n = cfg->pc_tol > 0 ? n3r : n3; /* components in the test */
             for (;;) {
-                if (pc_err < 1e-16) break;
+                if (pc_err < pc_tol) break;
                 if (it > 2 && pc_err_last <= pc_err) break;
                 if (it >= 12) { st->pc_maxed++; break; }
                 pc_err_last = pc_err;
@@ -345,10 +350,12 @@ int orc_integrate(const orc_ephem *eph, const orc_opts *o, orc_ias15_cfg *cfg, o
                             b[4][k] += tmp * C[19];
                             b[5][k] += tmp * C[20];
                             b[6][k] += tmp;
-                            const double ak = fabs(B.at[k]);
-                            if (isnormal(ak) && ak > maxak) maxak = ak;
-                            const double db6 = fabs(tmp);
-                            if (isnormal(db6) && db6 > maxdb6) maxdb6 = db6;
+                            if (k < pc_n) {
+                                const double ak = fabs(B.at[k]);
+                                if (isnormal(ak) && ak > maxak) maxak = ak;
+                                const double db6 = fabs(tmp);
+                                if (isnormal(db6) && db6 > maxdb6) maxdb6 = db6;
+                            }
                         }
                         }
                     }

@@ -88,6 +88,9 @@ typedef struct {
     double epsilon, min_dt, dt; /* dt: initial (then proposed) step */
     int exact_finish;
     long max_steps;
+    /* predictor-corrector stop: 0 = max|db6| / max|a| over every component below 1e-16;
+     * > 0 = the real particles' max|db6| / max|a| below pc_tol (the product's option) */
+    double pc_tol;
 } orc_ias15_cfg;
 
 typedef struct {

@@ -1,9 +1,9 @@
 """Parity at the benchmark configurations' full sizes.
 
 The 1 m bar is stated for orbits without close approaches.  Every system is held to it unless it
-is outside the bar *and* its own measured trajectory comes within max(0.1 au, 3 Hill radii) of a
-planet (tests/encounters.py); the exemptions are printed, and they must stay under 5% of the
-configuration.
+is outside the bar *and* its own measured trajectory comes within 3 Hill radii, or 0.1 au
+(planets) / 0.05 au (asteroids), of a perturber (tests/encounters.py); the exemptions are
+printed, and they must stay under 5% of the configuration.
 
 * Configuration 2 in full (4 096 main-belt systems x (1 + 6 variational), 3 650 days) against
   the oracle, system by system: <= 1 m in position, variational states to 1e-6.
