@@ -137,6 +137,22 @@ bool flatten_ascii_column(const ascii_s* a, int col, double scale, double jd_ref
     return true;
 }
 
+// The device copy of the ephemeris on the current device if one is resident (nothing is built
+// or allocated): ASSIST_BATCH_ERROR_UNSUPPORTED otherwise.
+int lookup_device_ephem(const assist_ephem* e, const DevEphem** out) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!e || (!e->spk_planets && !e->ascii_planets)) return ASSIST_ERROR_EPHEM_FILE;
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return ASSIST_BATCH_ERROR_DEVICE;
+    const DeviceEphemCache* c =
+        (const DeviceEphemCache*)(e->spk_planets ? e->spk_planets->device_cache : e->ascii_planets->device_cache);
+    if (c && c->device == dev && c->jd_ref == e->jd_ref && c->asteroids == e->spk_asteroids) {
+        *out = c->d_ephem;
+        return ASSIST_SUCCESS;
+    }
+    return ASSIST_BATCH_ERROR_UNSUPPORTED;
+}
+
 // Builds (or reuses) the device copy of the ephemeris on the current device.
 int get_device_ephem(const assist_ephem* e, const DevEphem** out, DevEphem* host_copy) {
     std::lock_guard<std::mutex> lock(g_mutex);
@@ -904,6 +920,76 @@ static int integrate_device_locked(const struct assist_ephem* ephem, const struc
 }
 
 extern "C" {
+
+int assist_batch_integrate_device_noalloc(const struct assist_ephem* ephem, const struct assist_extras* opts,
+                                          const struct assist_batch_config* cfg_in, int n, int n_var,
+                                          const double* d_t0, const double* d_t1, double* d_states,
+                                          const double* d_params, int* d_status, long long* d_steps,
+                                          long long* d_evals, void* stream) {
+    if (cfg_in && cfg_in->size != sizeof(struct assist_batch_config)) return ASSIST_BATCH_ERROR_ARG;
+    if (n < 0 || n_var < 0 || n_var > 7 || !d_t0 || !d_t1 || !d_states || !d_status) return ASSIST_BATCH_ERROR_ARG;
+    struct assist_batch_config cfg;
+    if (cfg_in) cfg = *cfg_in;
+    else assist_batch_config_default(&cfg);
+    if (cfg.use_graph) return ASSIST_BATCH_ERROR_UNSUPPORTED;  // capturing a graph allocates
+    if (n == 0) return ASSIST_SUCCESS;
+    IntegrateArgs a;
+    memset(&a, 0, sizeof a);
+    int rc = make_force_opts(opts, &a.fo);
+    if (rc) return rc;
+    rc = lookup_device_ephem(ephem, &a.E);
+    if (rc == ASSIST_BATCH_ERROR_UNSUPPORTED)
+        fprintf(stderr, "(ASSIST) assist_batch_integrate_device_noalloc: the ephemeris is not resident on this GPU; "
+                        "make it resident with any other batched call first.\n");
+    if (rc) return rc;
+    const int nv = 1 + n_var;
+    const bool mixed = cfg.precision == ASSIST_PRECISION_MIXED;
+    const bool ext = ext_model(a.fo);
+    if (ext && mixed) return ASSIST_BATCH_ERROR_UNSUPPORTED;
+    const int G = lanes_for(nv, cfg.lanes_per_system);
+    const int threads = cfg.block_size > 0 ? cfg.block_size : kIas15Block;
+    if (G < 0 || threads % 64 != 0 || threads > kIas15Block || threads % G != 0) return ASSIST_BATCH_ERROR_ARG;
+    if (cfg.waves_per_simd != 0 && cfg.waves_per_simd != 1) return ASSIST_BATCH_ERROR_UNSUPPORTED;
+    a.io.epsilon = cfg.epsilon;
+    a.io.min_dt = cfg.min_dt;
+    a.io.dt_initial = cfg.dt_initial;
+    a.io.max_steps = cfg.max_steps;
+    a.io.pc_tol = corrector_tolerance(cfg);
+    a.nsys = n;
+    a.nvec = nv;
+    a.t0 = d_t0;
+    a.t1 = d_t1;
+    a.states = d_states;
+    a.params = d_params;
+    a.status = d_status;
+    a.steps = d_steps;
+    a.evals = d_evals;
+    std::lock_guard<std::mutex> lock(g_launch_mutex);
+    double* scr = nullptr;
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (!hip_ok(noalloc_scratch(&scr), "hipGetSymbolAddress(scratch)") || !hip_ok(hipGetDevice(&dev), "hipGetDevice") ||
+        !hip_ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute") ||
+        !hip_ok(ias15_occupancy(G, mixed, 1, ext, nv, threads, &per_cu), "occupancy"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    if (per_cu < 1) per_cu = 1;
+    if (cfg.blocks_per_cu > 0 && cfg.blocks_per_cu < per_cu) per_cu = cfg.blocks_per_cu;
+    // the grid: one pass of the persistent kernel over at most kNoAllocLanes resident lanes (the
+    // b/e records of the static scratch); the work queue hands every system to them
+    long long blocks = ((long long)n * G + threads - 1) / threads;
+    if (blocks > (long long)per_cu * ncu) blocks = (long long)per_cu * ncu;
+    if (blocks > kNoAllocLanes / threads) blocks = kNoAllocLanes / threads;
+    if (blocks < 1) return ASSIST_BATCH_ERROR_ARG;
+    a.queue = (int*)scr;
+    a.scratch = scr + kNoAllocHeadWords;
+    a.ecache = nullptr;  // no substep cache: same results, computed again in each iteration
+    const int total_groups = (int)blocks * threads / G;
+    hipStream_t s = (hipStream_t)stream;
+    if (!hip_ok(launch_set_counter(a.queue, total_groups, s), "launch(set_counter)") ||
+        !hip_ok(launch_ias15(a, G, mixed, 1, (int)blocks, threads, s), "launch(ias15)") ||
+        !hip_ok(hipStreamSynchronize(s), "ias15"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    return ASSIST_SUCCESS;
+}
 
 int assist_batch_get_particles_device(const struct assist_ephem* ephem, int n, const int* d_body, const double* d_t,
                                       double* d_out, int* d_status, void* stream) {

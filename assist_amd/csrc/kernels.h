@@ -124,6 +124,14 @@ hipError_t ias15_occupancy(int G, bool mixed, int waves, bool ext, int nvec, int
 inline bool ext_model(const ForceOpts& f) { return f.geocentric != 0 || ((f.forces & 0x40) && f.gr_eih_sources > 1); }
 size_t ias15_lds_bytes(int G, int nvec, int threads, int waves, bool ext);
 hipError_t launch_set_counter(int* q, int v, hipStream_t s);
+// Scratch of the allocation-free entry point: a static device array of the library (22 MB per
+// GPU): kNoAllocHeadWords doubles for the work-queue counter, then the b/e records of up to
+// kNoAllocLanes resident lanes (42 doubles each).  *ptr receives its device address on the
+// current device (looked up once per device).
+constexpr int kNoAllocLanes = 65536;
+constexpr int kNoAllocHeadWords = 32;
+constexpr int kNoAllocMaxDevices = 64;
+hipError_t noalloc_scratch(double** ptr);
 // Queue order of large batches: order[0..n) = system indices sorted by span |t1 - t0|, longest
 // first, and among equal spans by the real particle's perihelion distance, smallest first (stable,
 // so equal keys keep their index order).  Buffers: keys/keys_alt n words, vals n ints.

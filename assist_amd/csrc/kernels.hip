@@ -1068,6 +1068,35 @@ hipError_t ias15_occupancy(int G, bool mixed, int waves, bool ext, int nvec, int
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, threads, lds);
 }
 
+// ---- scratch of the allocation-free integration entry point -------------------------------
+// A file-scope device array: every GPU that runs this library holds its own copy until the
+// process ends.  The work-queue counter sits in its first 256 bytes, then the b/e records (42
+// doubles per lane) of at most kNoAllocLanes resident lanes.  Host code never passes the array's
+// name to a kernel (on the host it names a stand-in, not device memory): it passes the device
+// address that hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocScratchWords = kNoAllocHeadWords + (size_t)kNoAllocLanes * 42;
+static __device__ __align__(256) double scratch[kNoAllocScratchWords];
+
+hipError_t noalloc_scratch(double** ptr) {
+    // one address per device, looked up once with that device current
+    static std::mutex m;
+    static double* addr[kNoAllocMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(m);
+    if (!addr[dev]) {
+        double* p = nullptr;
+        e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    return hipSuccess;
+}
+
 // Sets the work-queue counter (a kernel, so the sequence can be captured in a graph).
 __global__ void set_counter_kernel(int* q, int v) { *q = v; }
 

@@ -385,6 +385,24 @@ int assist_batch_integrate_device_ex(const struct assist_ephem* ephem, const str
                                      long long* d_steps, long long* d_evals, void* stream,
                                      struct assist_batch_stats* stats, const struct assist_batch_extra* ex);
 
+/* Allocation-free variant of assist_batch_integrate_device (no counterpart in the reference):
+ * for callers that cannot allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  It calls no hipMalloc, hipFree or other allocating
+ * routine: its scratch (the work-queue counter and the b/e records of at most 65 536 resident
+ * lanes, 22 MB) is a static device array of the library, which every GPU the library runs on holds
+ * until the process ends, and the grid is capped to it.  It runs without the substep cache, the
+ * autotuner, the queue order, graph capture and timing (same results bit for bit; use_graph = 1
+ * returns ASSIST_BATCH_ERROR_UNSUPPORTED).  The ephemeris must already be resident on the GPU
+ * (any other batched call with it makes it so; otherwise ASSIST_BATCH_ERROR_UNSUPPORTED).  Calls
+ * that overlap on one GPU share that array and race: the library's lock serialises calls from
+ * threads of one process, but not from several processes on one GPU.  Returns after the kernel
+ * completes on `stream`. */
+int assist_batch_integrate_device_noalloc(const struct assist_ephem* ephem, const struct assist_extras* opts,
+                                          const struct assist_batch_config* cfg, int n, int n_var,
+                                          const double* d_t0, const double* d_t1, double* d_states,
+                                          const double* d_params, int* d_status, long long* d_steps,
+                                          long long* d_evals, void* stream);
+
 /* Dense output with host buffers: out_offset[n + 1], out_t[out_offset[n]], out_states
  * [out_offset[n]][1 + n_var][6]; the output times of each system are checked to be sorted in one
  * direction from t0[i] (ASSIST_BATCH_ERROR_ARG otherwise).  states receive the end of each
