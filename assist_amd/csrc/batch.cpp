@@ -139,7 +139,7 @@ bool flatten_ascii_column(const ascii_s* a, int col, double scale, double jd_ref
 
 // The device copy of the ephemeris on the current device if one is resident (nothing is built
 // or allocated): ASSIST_BATCH_ERROR_UNSUPPORTED otherwise.
-int lookup_device_ephem(const assist_ephem* e, const DevEphem** out) {
+int lookup_device_ephem(const assist_ephem* e, const DevEphem** out, DevEphem* host_copy) {
     std::lock_guard<std::mutex> lock(g_mutex);
     if (!e || (!e->spk_planets && !e->ascii_planets)) return ASSIST_ERROR_EPHEM_FILE;
     int dev = 0;
@@ -148,6 +148,7 @@ int lookup_device_ephem(const assist_ephem* e, const DevEphem** out) {
         (const DeviceEphemCache*)(e->spk_planets ? e->spk_planets->device_cache : e->ascii_planets->device_cache);
     if (c && c->device == dev && c->jd_ref == e->jd_ref && c->asteroids == e->spk_asteroids) {
         *out = c->d_ephem;
+        *host_copy = c->host;
         return ASSIST_SUCCESS;
     }
     return ASSIST_BATCH_ERROR_UNSUPPORTED;
@@ -529,7 +530,9 @@ int run_ias15(IntegrateArgs a, bool mixed, const LaunchShape& sh, int blocks_per
     int dev = 0, ncu = 0, per_cu = 0;
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice") ||
         !hip_ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute") ||
-        !hip_ok(ias15_occupancy(sh.G, mixed, sh.waves, ext_model(a.fo), a.nvec, sh.threads, &per_cu), "occupancy"))
+        !hip_ok(ias15_occupancy(sh.G, mixed, sh.waves, ext_model(a.fo), ias15_snap(a), a.nvec, sh.threads,
+                                &per_cu),
+                "occupancy"))
         return ASSIST_BATCH_ERROR_DEVICE;
     if (per_cu < 1) per_cu = 1;
     if (blocks_per_cu_cap > 0 && blocks_per_cu_cap < per_cu) per_cu = blocks_per_cu_cap;
@@ -621,27 +624,25 @@ int tuned_shape(const IntegrateArgs& base, bool mixed, hipStream_t s, LaunchShap
     for (int G : {1, 8}) {
         if (G < base.nvec) continue;
         for (int threads : {256, 128}) {
-            for (int waves : {1}) {
-                LaunchShape sh;
-                sh.G = G;
-                sh.threads = threads;
-                sh.waves = waves;
-                // each shape runs twice and keeps the faster time: a kernel's first launch can
-                // include loading its code, which the events around it would count
-                float ms = 1e30f;
-                for (int rep = 0; rep < 2; rep++) {
-                    if (!hip_ok(hipMemcpyAsync(st.p, base.states, sw * sizeof(double), hipMemcpyDeviceToDevice, s),
-                                "hipMemcpyAsync"))
-                        return ASSIST_BATCH_ERROR_DEVICE;
-                    float t_ms = 0.f;
-                    const int rc = run_ias15(a, mixed, sh, 0, s, &t_ms, nullptr);
-                    if (rc) return rc;
-                    if (t_ms < ms) ms = t_ms;
-                }
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best = sh;
-                }
+            LaunchShape sh;
+            sh.G = G;
+            sh.threads = threads;
+            sh.waves = 1;
+            // each shape runs twice and keeps the faster time: a kernel's first launch can
+            // include loading its code, which the events around it would count
+            float ms = 1e30f;
+            for (int rep = 0; rep < 2; rep++) {
+                if (!hip_ok(hipMemcpyAsync(st.p, base.states, sw * sizeof(double), hipMemcpyDeviceToDevice, s),
+                            "hipMemcpyAsync"))
+                    return ASSIST_BATCH_ERROR_DEVICE;
+                float t_ms = 0.f;
+                const int rc = run_ias15(a, mixed, sh, 0, s, &t_ms, nullptr);
+                if (rc) return rc;
+                if (t_ms < ms) ms = t_ms;
+            }
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = sh;
             }
         }
     }
@@ -727,8 +728,10 @@ int assist_batch_forces(const struct assist_ephem* ephem, const struct assist_ex
     int rc = make_force_opts(opts, &a.fo);
     if (rc) return rc;
     if (ext_model(a.fo) && precision == ASSIST_PRECISION_MIXED) return ASSIST_BATCH_ERROR_UNSUPPORTED;
-    rc = get_device_ephem(ephem, &a.E, nullptr);
+    DevEphem H;
+    rc = get_device_ephem(ephem, &a.E, &H);
     if (rc) return rc;
+    coverage_window(H, a.fo);
     const int nv = 1 + n_var;
     const int G = lanes_for(nv, 0);
     if (G < 0) return ASSIST_BATCH_ERROR_ARG;
@@ -819,8 +822,10 @@ static int integrate_device_locked(const struct assist_ephem* ephem, const struc
     memset(&a, 0, sizeof a);
     int rc = make_force_opts(opts, &a.fo);
     if (rc) return rc;
-    rc = get_device_ephem(ephem, &a.E, nullptr);
+    DevEphem H;
+    rc = get_device_ephem(ephem, &a.E, &H);
     if (rc) return rc;
+    coverage_window(H, a.fo);
     const int nv = 1 + n_var;
     const bool mixed = cfg.precision == ASSIST_PRECISION_MIXED;
     a.io.epsilon = cfg.epsilon;
@@ -937,11 +942,13 @@ int assist_batch_integrate_device_noalloc(const struct assist_ephem* ephem, cons
     memset(&a, 0, sizeof a);
     int rc = make_force_opts(opts, &a.fo);
     if (rc) return rc;
-    rc = lookup_device_ephem(ephem, &a.E);
+    DevEphem H;
+    rc = lookup_device_ephem(ephem, &a.E, &H);
     if (rc == ASSIST_BATCH_ERROR_UNSUPPORTED)
         fprintf(stderr, "(ASSIST) assist_batch_integrate_device_noalloc: the ephemeris is not resident on this GPU; "
                         "make it resident with any other batched call first.\n");
     if (rc) return rc;
+    coverage_window(H, a.fo);
     const int nv = 1 + n_var;
     const bool mixed = cfg.precision == ASSIST_PRECISION_MIXED;
     const bool ext = ext_model(a.fo);
@@ -969,7 +976,7 @@ int assist_batch_integrate_device_noalloc(const struct assist_ephem* ephem, cons
     int dev = 0, ncu = 0, per_cu = 0;
     if (!hip_ok(noalloc_scratch(&scr), "hipGetSymbolAddress(scratch)") || !hip_ok(hipGetDevice(&dev), "hipGetDevice") ||
         !hip_ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute") ||
-        !hip_ok(ias15_occupancy(G, mixed, 1, ext, nv, threads, &per_cu), "occupancy"))
+        !hip_ok(ias15_occupancy(G, mixed, 1, ext, false, nv, threads, &per_cu), "occupancy"))
         return ASSIST_BATCH_ERROR_DEVICE;
     if (per_cu < 1) per_cu = 1;
     if (cfg.blocks_per_cu > 0 && cfg.blocks_per_cu < per_cu) per_cu = cfg.blocks_per_cu;

@@ -74,7 +74,24 @@ struct ForceOpts {
     double alpha, nk, nm, nn, r0;  // Marsden g(r) shape
     int geocentric;      // 1: particle states relative to the Earth (reference src/forces.c:72-87)
     int pad_;
+    // coverage window of a force evaluation (days from jd_ref): the planet file's, narrowed to the
+    // asteroid file's when a direct term is on and asteroids are loaded (set with the ephemeris)
+    double t_lo, t_hi;
 };
+
+// Coverage window of a force evaluation.  The reference reports coverage errors from its direct
+// term (reference src/forces.c:315-321, 362-367), whose variational loop fetches the first asteroid
+// whatever the force flags, and from the geocentric frame (:76-80): the asteroid window therefore
+// applies when a direct term is on and an asteroid file is loaded.  The planet window always
+// applies: outside it the reference's other terms would use unset ephemeris values.
+ASSIST_HD inline void coverage_window(const DevEphem& E, ForceOpts& O) {
+    O.t_lo = E.t_beg;
+    O.t_hi = E.t_end;
+    if ((O.forces & 0x07) && E.n_ast > 0) {  // ASSIST_FORCE_SUN | _PLANETS | _ASTEROIDS
+        O.t_lo = O.t_lo > E.ta_beg ? O.t_lo : E.ta_beg;
+        O.t_hi = O.t_hi < E.ta_end ? O.t_hi : E.ta_end;
+    }
+}
 
 // IAS15 configuration for the batched integrator.
 struct IntegratorOpts {

@@ -192,19 +192,10 @@ struct ChebOut {
 struct alignas(16) Pair { double x, y; };
 
 // The device evaluates all kCoefPad terms of the zero-padded record with every load issued
-// up front; the operation-count instantiation (D != double) counts the file's P terms.
-template <typename D, bool Vel, bool Acc = false>
-AHD ChebOut<D> cheb_eval(const DevSlot& S, D t) {
-    const D fi = (D(S.c_init) + t) * D(S.inv_intlen);
-    int idx = (int)value_of(fi);
-    idx = idx < 0 ? 0 : (idx >= S.nrec ? S.nrec - 1 : idx);
-#if defined(__HIP_DEVICE_COMPILE__)
-    // the records live in global memory: address them as such (global, not flat, loads)
-    typedef __attribute__((address_space(1))) const double GlobalDouble;
-    GlobalDouble* rec = (GlobalDouble*)S.rec + (size_t)idx * kRecWords;
-#else
-    const double* rec = S.rec + (size_t)idx * kRecWords;
-#endif
+// up front; the operation-count instantiation (D != double) counts the file's P terms.  `rec`
+// points at the record (in global memory on the device, in host memory otherwise).
+template <typename D, bool Vel, bool Acc, typename Ptr>
+AHD ChebOut<D> cheb_series(const DevSlot& S, Ptr rec, D t) {
     double c[kRecWords];
 #pragma unroll
     for (int k = 0; k < kRecWords; k += 2) {
@@ -274,6 +265,27 @@ AHD ChebOut<D> cheb_eval(const DevSlot& S, D t) {
     o.y = y;
     o.z = w;
     return o;
+}
+
+// Record index of slot S at time t (clamped to the records present).
+template <typename D>
+AHD int record_index(const DevSlot& S, D t) {
+    const D fi = (D(S.c_init) + t) * D(S.inv_intlen);
+    const int idx = (int)value_of(fi);
+    return idx < 0 ? 0 : (idx >= S.nrec ? S.nrec - 1 : idx);
+}
+
+template <typename D, bool Vel, bool Acc = false>
+AHD ChebOut<D> cheb_eval(const DevSlot& S, D t) {
+    const int idx = record_index(S, t);
+#if defined(__HIP_DEVICE_COMPILE__)
+    // the records live in global memory: address them as such (global, not flat, loads)
+    typedef __attribute__((address_space(1))) const double GlobalDouble;
+    GlobalDouble* rec = (GlobalDouble*)S.rec + (size_t)idx * kRecWords;
+#else
+    const double* rec = S.rec + (size_t)idx * kRecWords;
+#endif
+    return cheb_series<D, Vel, Acc>(S, rec, t);
 }
 
 // ---- substep ephemeris cache and wave-wide Chebyshev rounds (8-lane groups) ---------------
@@ -926,14 +938,8 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
     constexpr int NRV = Ext ? NR : 1;  // rounds with velocities and accelerations kept
     const int forces = O.forces;
     const double tv = value_of(t);
-    // The reference reports coverage errors from its direct term (reference src/forces.c:315-321,
-    // 362-367), whose variational loop fetches the first asteroid whatever the force flags, and
-    // from the geocentric frame (:76-80).  The asteroid window therefore applies when a direct
-    // term is on and an asteroid file is loaded.  The planet window always applies: outside it
-    // the reference's other terms would use unset ephemeris values.
-    bool covered = tv >= E.t_beg && tv <= E.t_end;
-    if ((O.forces & (F_SUN | F_PLANETS | F_ASTEROIDS)) && E.n_ast > 0) covered = covered && tv >= E.ta_beg && tv <= E.ta_end;
-    out.status = covered ? kOk : kCoverage;
+    // (the window is coverage_window's, precomputed with the ephemeris)
+    out.status = (tv >= O.t_lo && tv <= O.t_hi) ? kOk : kCoverage;
     const bool eih = (forces & F_GR_EIH) && O.gr_eih_sources > 0;
     const bool want_sun_vel = eih || ((forces & F_NONGRAV) && ng) || (forces & F_GR_SIMPLE);
 

@@ -119,7 +119,11 @@ hipError_t launch_forces(const ForcesArgs& a, int G, bool mixed, hipStream_t s);
 // waves: the kernel variant compiled for 1 or 2 waves per SIMD (register budget 512 or 256).
 hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int waves, int blocks, int threads, hipStream_t s,
                         bool set_attributes = true);
-hipError_t ias15_occupancy(int G, bool mixed, int waves, bool ext, int nvec, int threads, int* blocks_per_cu);
+hipError_t ias15_occupancy(int G, bool mixed, int waves, bool ext, bool snap, int nvec, int threads,
+                           int* blocks_per_cu);
+// Whether a launch uses the checkpoint, pause or step-archive features (kernels without them
+// exist for the hot shapes).
+inline bool ias15_snap(const IntegrateArgs& a) { return a.snap_in || a.snap_out || a.t_pause || a.archive; }
 // The extended force model (geocentric frame or several EIH sources) has kernels of its own.
 inline bool ext_model(const ForceOpts& f) { return f.geocentric != 0 || ((f.forces & 0x40) && f.gr_eih_sources > 1); }
 size_t ias15_lds_bytes(int G, int nvec, int threads, int waves, bool ext);

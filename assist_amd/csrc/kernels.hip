@@ -124,7 +124,9 @@ __device__ __forceinline__ void predict_e(const double br[7], double q, double e
     e[6] = q7 * br[6];
 }
 
-template <int G, bool Mixed, int W, bool Ext = false>
+// Snap: the kernel carries the checkpoint, pause and step-archive features (the hot shapes are
+// also compiled without them, for launches that use none).
+template <int G, bool Mixed, int W, bool Ext = false, bool Snap = true>
 __global__ void __launch_bounds__(kIas15Block) __attribute__((amdgpu_waves_per_eu(W, W)))
 ias15_kernel(IntegrateArgs args) {
     // The arguments are read through the kernel-argument segment pointer, which is
@@ -219,7 +221,7 @@ ias15_kernel(IntegrateArgs args) {
         }
     };
     auto archive_snap = [&]() {
-        if (ka->archive) {
+        if (Snap && ka->archive) {
             if (nrec < ka->archive_cap)
                 write_snap(ka->archive + ((size_t)sys * ka->archive_cap + nrec) * snap_words(nv));
             nrec++;
@@ -261,7 +263,7 @@ ias15_kernel(IntegrateArgs args) {
         status = 0;
         steps = evals = rejected = 0;
         last = 0;
-        if (ka->snap_in) {
+        if (Snap && ka->snap_in) {
             // resume: the state, steps and counts of the snapshot, and the last step's b and e
             // for the prediction of the next step (the first evaluation recomputes a0)
             const double* sn = ka->snap_in + (size_t)s * snap_words(nv);
@@ -290,8 +292,8 @@ ias15_kernel(IntegrateArgs args) {
 #pragma unroll
             for (int c = 0; c < 3; c++) { st[c] = x0[c]; st[3 + c] = v0[c]; }
         }
-        if (ka->snap_out) write_snap(ka->snap_out + (size_t)sys * snap_words(nv));
-        if (lane == 0 && ka->archive_count) ka->archive_count[sys] = nrec;
+        if (Snap && ka->snap_out) write_snap(ka->snap_out + (size_t)sys * snap_words(nv));
+        if (Snap && lane == 0 && ka->archive_count) ka->archive_count[sys] = nrec;
         if (lane == 0) {
             if (ka->status) ka->status[sys] = status;
             if (ka->steps) ka->steps[sys] = steps;
@@ -645,7 +647,7 @@ ias15_kernel(IntegrateArgs args) {
         if (bad) { status = kNonFinite; finish(); continue; }
         archive_snap();
         if (last || (ka->out_t && cur >= cend)) { finish(); continue; }
-        if (ka->t_pause && (t - ka->t_pause[sys]) * dt_last >= 0) { status = kPaused; finish(); continue; }
+        if (Snap && ka->t_pause && (t - ka->t_pause[sys]) * dt_last >= 0) { status = kPaused; finish(); continue; }
         if (steps >= ka->io.max_steps) { status = kMaxSteps; finish(); continue; }
         phase = false;
     }
@@ -1011,12 +1013,21 @@ hipError_t launch_forces(const ForcesArgs& a, int G, bool mixed, hipStream_t s) 
     }
 }
 
-template <int G, bool M, int W, bool X = false>
+template <int G, bool M, int W, bool X = false, bool S = true>
 static const void* ias15_fn() {
-    return (const void*)ias15_kernel<G, M, W, X>;
+    return (const void*)ias15_kernel<G, M, W, X, S>;
 }
 
-static const void* ias15_lookup(int G, bool mixed, int waves, bool ext) {
+static const void* ias15_lookup(int G, bool mixed, int waves, bool ext, bool snap) {
+    if (!snap && !ext && waves == 1) {  // the hot shapes, without the checkpoint features
+        switch (G * 2 + (mixed ? 1 : 0)) {
+        case 2: return ias15_fn<1, false, 1, false, false>();
+        case 3: return ias15_fn<1, true, 1, false, false>();
+        case 16: return ias15_fn<8, false, 1, false, false>();
+        case 17: return ias15_fn<8, true, 1, false, false>();
+        default: break;
+        }
+    }
     if (ext) {  // extended force model: FP64, 1-wave variant
         if (mixed || waves != 1) return nullptr;
         switch (G) {
@@ -1045,7 +1056,7 @@ size_t ias15_lds_bytes(int G, int nvec, int threads, int waves, bool ext) {
 
 hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int waves, int blocks, int threads, hipStream_t s,
                         bool set_attributes) {
-    const void* f = ias15_lookup(G, mixed, waves, ext_model(a.fo));
+    const void* f = ias15_lookup(G, mixed, waves, ext_model(a.fo), ias15_snap(a));
     if (!f) return hipErrorInvalidValue;
     const size_t lds = ias15_lds_bytes(G, a.nvec, threads, waves, ext_model(a.fo));
     hipError_t e = hipSuccess;
@@ -1059,8 +1070,9 @@ hipError_t launch_ias15(const IntegrateArgs& a, int G, bool mixed, int waves, in
     return hipGetLastError();
 }
 
-hipError_t ias15_occupancy(int G, bool mixed, int waves, bool ext, int nvec, int threads, int* blocks_per_cu) {
-    const void* f = ias15_lookup(G, mixed, waves, ext);
+hipError_t ias15_occupancy(int G, bool mixed, int waves, bool ext, bool snap, int nvec, int threads,
+                           int* blocks_per_cu) {
+    const void* f = ias15_lookup(G, mixed, waves, ext, snap);
     if (!f) return hipErrorInvalidValue;
     const size_t lds = ias15_lds_bytes(G, nvec, threads, waves, ext);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

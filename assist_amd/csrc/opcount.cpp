@@ -115,6 +115,7 @@ static Counts count_eval(const DevEphem& E, int forces, int nvec, bool ng) {
     O.forces = forces;
     O.gr_eih_sources = 1;
     O.alpha = 1.0; O.nk = 0.0; O.nm = 2.0; O.nn = 5.093; O.r0 = 1.0;
+    coverage_window(E, O);
     const Cnt xr[3] = {Cnt(2.1), Cnt(-0.7), Cnt(0.3)}, vr[3] = {Cnt(0.002), Cnt(0.009), Cnt(0.001)};
     const double A[3] = {1e-12, 1e-13, 1e-14};
     g_ops64 = g_ops32 = 0;
