@@ -102,6 +102,9 @@ struct IntegratorOpts {
     // predictor-corrector stop: 0 = IAS15's own test, max|db6| / max|a| over every vector of the
     // system below 1e-16; > 0 = the real particle's max|db6| / max|a| below pc_tol
     double pc_tol;
+    // > 0: an attempt also ends after its first pass when that pass's change (the same measure)
+    // is below pc_tol1
+    double pc_tol1;
 };
 
 enum SystemStatus : int {

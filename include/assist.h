@@ -280,6 +280,11 @@ struct assist_batch_config {
                              stops when the real particle's max|db6| / max|a| is below
                              epsilon / 100 (1e-16 when epsilon is 0).  > 0: that tolerance on the
                              real particle, in either precision.  < 0: IAS15's own test. */
+    double first_pass_tolerance; /* a step attempt also ends after its first predictor-corrector
+                             pass when that pass changed the real particle's b6 by less than
+                             this fraction of its acceleration (the same measure).  0 (default):
+                             by precision -- the mixed form 0.3 epsilon, FP64 none.  > 0: that
+                             tolerance, in either precision.  < 0: none. */
 };
 
 struct assist_batch_stats {

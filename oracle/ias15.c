@@ -9,7 +9,8 @@
  *  - predictor-corrector iterations until max|delta b6| / max|a| < 1e-16
  *    (at most 12; stop early when the error stops decreasing after 3); as an
  *    option of the product (cfg->pc_tol > 0), until the real particles' own
- *    max|delta b6| / max|a| is below pc_tol,
+ *    max|delta b6| / max|a| is below pc_tol, and (cfg->pc_tol1 > 0) after the
+ *    first pass of an attempt when that pass changed them by less than pc_tol1,
  *  - the global step-size criterion dt' = dt (eps / (max|b6| / max|a|))^(1/7)
  *    over the real particles (ASSIST selects this "legacy global" mode,
  *    reference src/assist.c:446), rejection when dt' < dt/4, growth <= 4x,
@@ -116,6 +117,7 @@ void orc_ias15_cfg_default(orc_ias15_cfg *c) {
     c->exact_finish = 1;
     c->max_steps = 100000000L;
     c->pc_tol = 0.0;
+    c->pc_tol1 = 0.0;
 }
 
 static inline void add_cs(double *p, double *cs, double in) {
@@ -241,6 +243,7 @@ int orc_integrate(const orc_ephem *eph, const orc_opts *o, orc_ias15_cfg *cfg, o
             const int pc_n = cfg->pc_tol > 0 ? n3r : n3; /* components in the test */
             for (;;) {
                 if (pc_err < pc_tol) break;
+                if (it == 1 && cfg->pc_tol1 > 0 && pc_err < cfg->pc_tol1) break;
                 if (it > 2 && pc_err_last <= pc_err) break;
                 if (it >= 12) { st->pc_maxed++; break; }
                 pc_err_last = pc_err;

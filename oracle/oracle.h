@@ -91,6 +91,9 @@ typedef struct {
     /* predictor-corrector stop: 0 = max|db6| / max|a| over every component below 1e-16;
      * > 0 = the real particles' max|db6| / max|a| below pc_tol (the product's option) */
     double pc_tol;
+    /* > 0: also stop after the first pass of an attempt when its change (the same measure) is
+     * below pc_tol1 (the product's first-pass option); 0 = no first-pass stop */
+    double pc_tol1;
 } orc_ias15_cfg;
 
 typedef struct {
