@@ -356,6 +356,11 @@ struct Workspace {
     int* order = nullptr;
     void* otemp = nullptr;
     size_t order_n = 0, otemp_bytes = 0;
+    // learned start: rows [n][8], features [n][16], network outputs [n][32]
+    double* prows = nullptr;
+    float* pX = nullptr;
+    float* pY = nullptr;
+    size_t pred_n = 0;
 };
 Workspace g_ws;
 
@@ -407,6 +412,21 @@ int ensure_order(int n) {
         return ASSIST_BATCH_ERROR_DEVICE;
     g_ws.order_n = (size_t)n;
     g_ws.otemp_bytes = tb;
+    return ASSIST_SUCCESS;
+}
+
+// Buffers of the learned start for n systems (the workspace of the current device must exist).
+int ensure_predict(size_t n) {
+    if (n <= g_ws.pred_n) return ASSIST_SUCCESS;
+    (void)hipFree(g_ws.prows); (void)hipFree(g_ws.pX); (void)hipFree(g_ws.pY);
+    g_ws.prows = nullptr;
+    g_ws.pX = g_ws.pY = nullptr;
+    g_ws.pred_n = 0;
+    if (!hip_ok(hipMalloc(&g_ws.prows, n * 8 * sizeof(double)), "hipMalloc(predict)") ||
+        !hip_ok(hipMalloc(&g_ws.pX, n * ASSIST_MLP_IN * sizeof(float)), "hipMalloc(predict)") ||
+        !hip_ok(hipMalloc(&g_ws.pY, n * ASSIST_MLP_OUT * sizeof(float)), "hipMalloc(predict)"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    g_ws.pred_n = n;
     return ASSIST_SUCCESS;
 }
 
@@ -1299,29 +1319,25 @@ int assist_batch_predict_start(const struct assist_ephem* ephem, struct assist_m
     const DevEphem* E = nullptr;
     int rc = get_device_ephem(ephem, &E, nullptr);
     if (rc) return rc;
-    const int chunk = assist_mlp_max_batch(net);
-    const size_t ns = (size_t)n;
-    DevBuf<double> rows;
-    DevBuf<float> X, Y;
-    if (!rows.alloc(ns * 8) || !X.alloc((size_t)chunk * ASSIST_MLP_IN) ||
-        !Y.alloc((size_t)chunk * ASSIST_MLP_OUT))
-        return ASSIST_BATCH_ERROR_DEVICE;
+    // (the buffers are the library's persistent workspace, shared with the integration entry points)
+    std::lock_guard<std::mutex> lock(g_launch_mutex);
+    rc = ensure_workspace(0);
+    if (rc) return rc;
+    rc = ensure_predict((size_t)n);
+    if (rc) return rc;
     // the first pass sees the last step as 1 day; later passes see the previous prediction
-    if (!hip_ok(launch_start_rows(n, 1 + n_var, d_t0, d_states, 1.0, rows.p, nullptr), "launch(start_rows)"))
+    if (!hip_ok(launch_start_rows(n, 1 + n_var, d_t0, d_states, 1.0, g_ws.prows, nullptr), "launch(start_rows)"))
         return ASSIST_BATCH_ERROR_DEVICE;
     for (int it = 0; it < iterations; it++) {
-        for (int off = 0; off < n; off += chunk) {
-            const int m = (n - off < chunk) ? n - off : chunk;
-            if (!hip_ok(launch_features(E, m, rows.p + (size_t)off * 8, 8, X.p, nullptr, nullptr), "launch(features)"))
-                return ASSIST_BATCH_ERROR_DEVICE;
-            rc = assist_mlp_forward(net, X.p, m, Y.p, nullptr);
-            if (rc) return rc;
-            const bool last = it == iterations - 1;
-            if (!hip_ok(launch_start_post(m, Y.p, d_t0 + off, d_t1 + off, min_dt, rows.p + (size_t)off * 8,
-                                          d_dt0 + off, (last && d_bw) ? d_bw + (size_t)off * 21 : nullptr, nullptr),
-                        "launch(start_post)"))
-                return ASSIST_BATCH_ERROR_DEVICE;
-        }
+        if (!hip_ok(launch_features(E, n, g_ws.prows, 8, g_ws.pX, nullptr, nullptr), "launch(features)"))
+            return ASSIST_BATCH_ERROR_DEVICE;
+        rc = assist_mlp_predict(net, g_ws.pX, n, g_ws.pY, nullptr);
+        if (rc) return rc;
+        const bool last = it == iterations - 1;
+        if (!hip_ok(launch_start_post(n, g_ws.pY, d_t0, d_t1, min_dt, g_ws.prows, d_dt0, (last && d_bw) ? d_bw : nullptr,
+                                      nullptr),
+                    "launch(start_post)"))
+            return ASSIST_BATCH_ERROR_DEVICE;
     }
     if (!hip_ok(hipDeviceSynchronize(), "predict_start")) return ASSIST_BATCH_ERROR_DEVICE;
     return ASSIST_SUCCESS;

@@ -181,6 +181,118 @@ __global__ void fill_kernel(float* p, size_t n, float v) {
     if (i < n) p[i] = v;
 }
 
+// ---- prediction: the four layers fused in one kernel ----------------------------------------
+// The forward pass for prediction keeps no activations: each wave takes 16 rows at a time
+// through all four layers, with the BF16 weights staged once per workgroup in LDS (transposed,
+// [out][in], so that a lane's four consecutive inputs of one output are one 8-byte read) and the
+// 16 x 128 activation tile in the wave's own LDS area, through which the MFMA result layout
+// (four rows of one column per lane) becomes the next operand layout (four columns of one row).
+// Every product is the same v_mfma_f32_16x16x16_bf16 over the same 16-wide slices of the
+// reduction in the same order as gemm_bf16_kernel, with the same bias, tanh and BF16 rounding,
+// so the outputs equal assist_mlp_forward's bit for bit.
+constexpr int kPIn = ASSIST_MLP_IN, kPH = ASSIST_MLP_HIDDEN, kPOut = ASSIST_MLP_OUT;
+constexpr int kPW1 = 0, kPW2 = kPW1 + kPH * kPIn, kPW3 = kPW2 + kPH * kPH, kPW4 = kPW3 + kPH * kPH;
+constexpr int kPWords = kPW4 + kPOut * kPH;  // BF16 weights in LDS
+constexpr int kPBias = 3 * kPH + kPOut;       // FP32 biases in LDS
+constexpr int kPWaves = 4;
+constexpr size_t kPLdsBytes = (size_t)kPWords * 2 + (size_t)kPBias * 4 + (size_t)kPWaves * 16 * kPH * 2;
+
+struct PredictArgs {
+    const uint16_t* p16;  // BF16 parameters (layer by layer: W[in][out] row-major, then b[out])
+    const float* p;       // FP32 parameters (the biases are read from here)
+    const float* X;       // [M][16]
+    float* Y;             // [M][32]
+    int M;
+};
+
+// One layer of a 16-row tile: out = act(A W + b) for K = KC * 16 inputs and NT * 16 outputs.
+// a[KC]: this lane's operand fragments (row l % 16, inputs 16 kc + 4 (l / 16) + 0..3).
+template <int KC, int NT, bool Last>
+__device__ __forceinline__ void predict_layer(const bf16x4 (&a)[KC], const uint16_t* Wt, const float* bias,
+                                              uint16_t* tile, float* Y, int row0, int M) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, q = (lane >> 4) << 2;
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        const uint16_t* w = Wt + (size_t)(nt * 16 + r) * (KC * 16) + q;
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++) {
+            const bf16x4 b = *(const bf16x4*)(w + kc * 16);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kc], b, acc, 0, 0, 0);
+        }
+        const int col = nt * 16 + r;
+        const float bc = bias[col];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float v = acc[j] + bc;
+            if constexpr (Last) {
+                const int row = row0 + q + j;
+                if (row < M) Y[(size_t)row * kPOut + col] = v;
+            } else {
+                tile[(q + j) * kPH + col] = f2bf(tanhf(v));
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) mlp_predict_kernel(PredictArgs g) {
+    extern __shared__ __align__(16) unsigned char s_pred[];
+    uint16_t* const sw = (uint16_t*)s_pred;
+    float* const sb = (float*)(s_pred + (size_t)kPWords * 2);
+    uint16_t* const tiles = (uint16_t*)(s_pred + (size_t)kPWords * 2 + (size_t)kPBias * 4);
+    // stage the weights transposed ([out][in]) and the biases
+    {
+        const int dims[5] = {kPIn, kPH, kPH, kPH, kPOut};
+        const int dst[4] = {kPW1, kPW2, kPW3, kPW4};
+        size_t src = 0;
+        int boff = 0;
+        for (int l = 0; l < 4; l++) {
+            const int K = dims[l], N = dims[l + 1];
+            for (int i = threadIdx.x; i < K * N; i += blockDim.x) {
+                const int k = i / N, n = i - k * N;
+                sw[dst[l] + n * K + k] = g.p16[src + i];
+            }
+            src += (size_t)K * N;
+            for (int i = threadIdx.x; i < N; i += blockDim.x) sb[boff + i] = g.p[src + i];
+            src += N;
+            boff += N;
+        }
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = lane & 15, q = (lane >> 4) << 2;
+    uint16_t* const tile = tiles + (size_t)wave * 16 * kPH;
+    const int ntiles = (g.M + 15) >> 4;
+    for (int t = blockIdx.x * kPWaves + wave; t < ntiles; t += gridDim.x * kPWaves) {
+        const int row0 = t << 4;
+        bf16x4 a1[1];
+        {
+            const int row = row0 + r;
+#pragma unroll
+            for (int j = 0; j < 4; j++) a1[0][j] = (short)(row < g.M ? f2bf(g.X[(size_t)row * kPIn + q + j]) : 0);
+        }
+        predict_layer<1, kPH / 16, false>(a1, sw + kPW1, sb, tile, nullptr, row0, g.M);
+        bf16x4 a[kPH / 16];
+        // (LDS operations of one wave complete in order; the fence keeps the compiler's order)
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+        for (int kc = 0; kc < kPH / 16; kc++) a[kc] = *(const bf16x4*)(tile + r * kPH + kc * 16 + q);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        predict_layer<kPH / 16, kPH / 16, false>(a, sw + kPW2, sb + kPH, tile, nullptr, row0, g.M);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+        for (int kc = 0; kc < kPH / 16; kc++) a[kc] = *(const bf16x4*)(tile + r * kPH + kc * 16 + q);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        predict_layer<kPH / 16, kPH / 16, false>(a, sw + kPW3, sb + 2 * kPH, tile, nullptr, row0, g.M);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+        for (int kc = 0; kc < kPH / 16; kc++) a[kc] = *(const bf16x4*)(tile + r * kPH + kc * 16 + q);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        predict_layer<kPH / 16, kPOut / 16, true>(a, sw + kPW4, sb + 3 * kPH, tile, g.Y, row0, g.M);
+    }
+}
+
 }  // namespace assist_gpu
 
 using namespace assist_gpu;
@@ -339,6 +451,33 @@ int assist_mlp_forward(struct assist_mlp* n, const float* d_X, int M, float* d_Y
         ld_in = kDims[l + 1];
     }
     return 0;
+}
+
+// Prediction forward pass of any number of rows: d_X [M][16] -> d_Y [M][32], keeping no
+// activations (the fused kernel above; outputs equal assist_mlp_forward's bit for bit).
+int assist_mlp_predict(struct assist_mlp* n, const float* d_X, int M, float* d_Y, void* stream) {
+    if (!n || !d_X || !d_Y || M < 0) return ASSIST_BATCH_ERROR_ARG;
+    if (M == 0) return 0;
+    if (!ok(hipFuncSetAttribute((const void*)mlp_predict_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kPLdsBytes),
+            "hipFuncSetAttribute(mlp_predict)"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    int dev = 0, ncu = 0;
+    if (!ok(hipGetDevice(&dev), "hipGetDevice") ||
+        !ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute"))
+        return ASSIST_BATCH_ERROR_DEVICE;
+    PredictArgs g;
+    g.p16 = n->p16;
+    g.p = n->p;
+    g.X = d_X;
+    g.Y = d_Y;
+    g.M = M;
+    // one workgroup per CU (the staged weights take half of its LDS), or fewer for small batches
+    const int tiles = (M + 15) / 16;
+    int blocks = (tiles + kPWaves - 1) / kPWaves;
+    if (blocks > ncu) blocks = ncu;
+    hipLaunchKernelGGL(mlp_predict_kernel, dim3(blocks), dim3(64 * kPWaves), kPLdsBytes, (hipStream_t)stream, g);
+    return ok(hipGetLastError(), "mlp predict") ? 0 : ASSIST_BATCH_ERROR_DEVICE;
 }
 
 // Forward and backward pass of the weighted squared error against d_T [M][32]; leaves the

@@ -47,6 +47,7 @@ clibassist.assist_mlp_set_params.argtypes = [C.c_void_p, C.c_void_p]
 clibassist.assist_mlp_get_params.argtypes = [C.c_void_p, C.c_void_p]
 clibassist.assist_mlp_set_output_weights.argtypes = [C.c_void_p, C.c_void_p]
 clibassist.assist_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+clibassist.assist_mlp_predict.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
 clibassist.assist_mlp_loss_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_float),
                                             C.c_void_p, C.c_void_p]
 clibassist.assist_mlp_adam_step.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
@@ -106,6 +107,10 @@ class StepModel:
     def forward(self, d_X: DeviceArray, m: int, d_Y: DeviceArray, row0: int = 0):
         _check(clibassist.assist_mlp_forward(self.ptr, C.c_void_p(d_X.ptr + row0 * N_IN * 4), int(m),
                                              C.c_void_p(d_Y.ptr + row0 * N_OUT * 4), None))
+
+    def predict(self, d_X: DeviceArray, m: int, d_Y: DeviceArray):
+        """The forward pass of the prediction path (any m, no activations kept)."""
+        _check(clibassist.assist_mlp_predict(self.ptr, C.c_void_p(d_X.ptr), int(m), C.c_void_p(d_Y.ptr), None))
 
     def loss_grad(self, d_X: DeviceArray, d_T: DeviceArray, m: int, row0: int = 0, want_grad: bool = False):
         loss = C.c_float(0.0)

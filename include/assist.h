@@ -472,6 +472,9 @@ int assist_mlp_get_params(const struct assist_mlp* net, float* host);
 int assist_mlp_set_output_weights(struct assist_mlp* net, const float* host_w);
 /* d_X[M][16] -> d_Y[M][32] (device pointers, M <= max_batch). */
 int assist_mlp_forward(struct assist_mlp* net, const float* d_X, int M, float* d_Y, void* stream);
+/* The same outputs, bit for bit, for any M, keeping no activations (one fused kernel: the
+ * prediction path of assist_batch_predict_start). */
+int assist_mlp_predict(struct assist_mlp* net, const float* d_X, int M, float* d_Y, void* stream);
 /* Forward and backward pass of the loss against d_T[M][32]; gradients stay on the device
  * (host_grad, if given, receives a copy) and *loss receives the loss. */
 int assist_mlp_loss_grad(struct assist_mlp* net, const float* d_X, const float* d_T, int M, float* loss,

@@ -51,6 +51,28 @@ def test_mlp_forward_matches_oracle(gpu):
     assert err.mean() <= 1e-3 * scale
 
 
+@pytest.mark.parametrize("m", [1, 301, 4096, 70001])
+def test_mlp_predict_equals_forward(gpu, m):
+    """The fused prediction kernel (all four layers in one launch, weights staged in LDS) gives
+    the forward pass's outputs bit for bit, on ragged row counts and with the shipped network;
+    it also takes batches beyond the network's workspace (70 001 rows against 65 536)."""
+    import assist_amd.learned as L
+    from assist_amd.batch import DeviceArray
+    rng = np.random.default_rng(40 + m)
+    X = rng.normal(0, 1, (m, 16)).astype(np.float32)
+    for net in (L.StepModel(max_batch=min(m, 65536), seed=5), L.StepModel.load(L.default_model_path(), max_batch=min(m, 65536))):
+        d_X, d_P = DeviceArray(X), DeviceArray(np.zeros((m, 32), np.float32))
+        net.predict(d_X, m, d_P)
+        yp = d_P.download()
+        yf = np.zeros_like(yp)
+        for lo in range(0, m, 65536):
+            hi = min(m, lo + 65536)
+            d_x, d_y = DeviceArray(X[lo:hi]), DeviceArray(np.zeros((hi - lo, 32), np.float32))
+            net.forward(d_x, hi - lo, d_y)
+            yf[lo:hi] = d_y.download()
+        np.testing.assert_array_equal(yp, yf)
+
+
 def test_mlp_gradients_match_oracle(gpu):
     import assist_amd.learned as L
     from assist_amd.batch import DeviceArray
