@@ -145,3 +145,23 @@ def test_configuration_3_full_size_against_oracle(gpu, ephem, oracle, model):
           % (a["evals"].sum(), o["evals"].sum(), (span > 5000).sum(), np.percentile(d[span > 5000], 99)))
     bar = np.maximum(1.0, span / 3650.0)
     bar_with_measured_exemptions(ephem, "configuration 3 in full vs oracle", d, t0, t1, states, params, bar=bar)
+
+
+def test_configuration_3_mixed_learned_full_size(gpu, ephem, model):
+    """Configuration 3 in full in the mixed form with the learned start (the tuner's launch shape)
+    against the FP64 form: every status 0, each system within 1 m per 10 years of its span unless
+    a close approach is measured on its own trajectory, and fewer evaluations."""
+    from assist_amd import batch
+    t0, t1, states, params = _workload("c3", model)
+    net = _net()
+    f = _run(ephem, t0, t1, states, params, batch.BatchConfig.default(min_dt=0.001, max_steps=200000))
+    m = _run(ephem, t0, t1, states, params,
+             batch.BatchConfig.default(precision=batch.PRECISION_MIXED, min_dt=0.001, max_steps=200000), net)
+    assert (f["status"] == 0).all() and (m["status"] == 0).all()
+    d = np.linalg.norm(m["states"][:, 0, :3] - f["states"][:, 0, :3], axis=-1) * AU_M
+    span = np.abs(t1 - t0)
+    print("configuration 3 in full, mixed+learned: %d evaluations against %d for FP64 (%.3fx); p99 |dr| %.3e m"
+          % (m["evals"].sum(), f["evals"].sum(), f["evals"].sum() / m["evals"].sum(), np.percentile(d, 99)))
+    assert m["evals"].sum() < f["evals"].sum()
+    bar_with_measured_exemptions(ephem, "configuration 3 in full, mixed+learned vs FP64", d, t0, t1, states, params,
+                                 bar=np.maximum(1.0, span / 3650.0))
