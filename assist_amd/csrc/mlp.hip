@@ -190,12 +190,17 @@ __global__ void fill_kernel(float* p, size_t n, float v) {
 // Every product is the same v_mfma_f32_16x16x16_bf16 over the same 16-wide slices of the
 // reduction in the same order as gemm_bf16_kernel, with the same bias, tanh and BF16 rounding,
 // so the outputs equal assist_mlp_forward's bit for bit.
+// (LDS rows are padded by 4 elements, 8 bytes: the 16 lanes that read one operand slice then
+// fall in distinct banks instead of all in one)
 constexpr int kPIn = ASSIST_MLP_IN, kPH = ASSIST_MLP_HIDDEN, kPOut = ASSIST_MLP_OUT;
-constexpr int kPW1 = 0, kPW2 = kPW1 + kPH * kPIn, kPW3 = kPW2 + kPH * kPH, kPW4 = kPW3 + kPH * kPH;
-constexpr int kPWords = kPW4 + kPOut * kPH;  // BF16 weights in LDS
-constexpr int kPBias = 3 * kPH + kPOut;       // FP32 biases in LDS
-constexpr int kPWaves = 4;
-constexpr size_t kPLdsBytes = (size_t)kPWords * 2 + (size_t)kPBias * 4 + (size_t)kPWaves * 16 * kPH * 2;
+constexpr int kPad = 4;
+constexpr int kPW1 = 0, kPW2 = kPW1 + kPH * (kPIn + kPad), kPW3 = kPW2 + kPH * (kPH + kPad),
+              kPW4 = kPW3 + kPH * (kPH + kPad);
+constexpr int kPWords = kPW4 + kPOut * (kPH + kPad);  // BF16 weights in LDS
+constexpr int kPBias = 3 * kPH + kPOut;                 // FP32 biases in LDS
+constexpr int kPWaves = 16;  // four waves per SIMD share one copy of the weights
+constexpr int kPTileRow = kPH + kPad;
+constexpr size_t kPLdsBytes = (size_t)kPWords * 2 + (size_t)kPBias * 4 + (size_t)kPWaves * 16 * kPTileRow * 2;
 
 struct PredictArgs {
     const uint16_t* p16;  // BF16 parameters (layer by layer: W[in][out] row-major, then b[out])
@@ -215,7 +220,7 @@ __device__ __forceinline__ void predict_layer(const bf16x4 (&a)[KC], const uint1
 #pragma unroll
     for (int nt = 0; nt < NT; nt++) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        const uint16_t* w = Wt + (size_t)(nt * 16 + r) * (KC * 16) + q;
+        const uint16_t* w = Wt + (size_t)(nt * 16 + r) * (KC * 16 + kPad) + q;
 #pragma unroll
         for (int kc = 0; kc < KC; kc++) {
             const bf16x4 b = *(const bf16x4*)(w + kc * 16);
@@ -230,13 +235,13 @@ __device__ __forceinline__ void predict_layer(const bf16x4 (&a)[KC], const uint1
                 const int row = row0 + q + j;
                 if (row < M) Y[(size_t)row * kPOut + col] = v;
             } else {
-                tile[(q + j) * kPH + col] = f2bf(tanhf(v));
+                tile[(q + j) * kPTileRow + col] = f2bf(tanhf(v));
             }
         }
     }
 }
 
-__global__ void __launch_bounds__(256) mlp_predict_kernel(PredictArgs g) {
+__global__ void __launch_bounds__(64 * kPWaves) mlp_predict_kernel(PredictArgs g) {
     extern __shared__ __align__(16) unsigned char s_pred[];
     uint16_t* const sw = (uint16_t*)s_pred;
     float* const sb = (float*)(s_pred + (size_t)kPWords * 2);
@@ -251,7 +256,7 @@ __global__ void __launch_bounds__(256) mlp_predict_kernel(PredictArgs g) {
             const int K = dims[l], N = dims[l + 1];
             for (int i = threadIdx.x; i < K * N; i += blockDim.x) {
                 const int k = i / N, n = i - k * N;
-                sw[dst[l] + n * K + k] = g.p16[src + i];
+                sw[dst[l] + n * (K + kPad) + k] = g.p16[src + i];
             }
             src += (size_t)K * N;
             for (int i = threadIdx.x; i < N; i += blockDim.x) sb[boff + i] = g.p[src + i];
@@ -262,7 +267,7 @@ __global__ void __launch_bounds__(256) mlp_predict_kernel(PredictArgs g) {
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = lane & 15, q = (lane >> 4) << 2;
-    uint16_t* const tile = tiles + (size_t)wave * 16 * kPH;
+    uint16_t* const tile = tiles + (size_t)wave * 16 * kPTileRow;
     const int ntiles = (g.M + 15) >> 4;
     for (int t = blockIdx.x * kPWaves + wave; t < ntiles; t += gridDim.x * kPWaves) {
         const int row0 = t << 4;
@@ -277,17 +282,17 @@ __global__ void __launch_bounds__(256) mlp_predict_kernel(PredictArgs g) {
         // (LDS operations of one wave complete in order; the fence keeps the compiler's order)
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 #pragma unroll
-        for (int kc = 0; kc < kPH / 16; kc++) a[kc] = *(const bf16x4*)(tile + r * kPH + kc * 16 + q);
+        for (int kc = 0; kc < kPH / 16; kc++) a[kc] = *(const bf16x4*)(tile + r * kPTileRow + kc * 16 + q);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         predict_layer<kPH / 16, kPH / 16, false>(a, sw + kPW2, sb + kPH, tile, nullptr, row0, g.M);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 #pragma unroll
-        for (int kc = 0; kc < kPH / 16; kc++) a[kc] = *(const bf16x4*)(tile + r * kPH + kc * 16 + q);
+        for (int kc = 0; kc < kPH / 16; kc++) a[kc] = *(const bf16x4*)(tile + r * kPTileRow + kc * 16 + q);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         predict_layer<kPH / 16, kPH / 16, false>(a, sw + kPW3, sb + 2 * kPH, tile, nullptr, row0, g.M);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 #pragma unroll
-        for (int kc = 0; kc < kPH / 16; kc++) a[kc] = *(const bf16x4*)(tile + r * kPH + kc * 16 + q);
+        for (int kc = 0; kc < kPH / 16; kc++) a[kc] = *(const bf16x4*)(tile + r * kPTileRow + kc * 16 + q);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         predict_layer<kPH / 16, kPOut / 16, true>(a, sw + kPW4, sb + 3 * kPH, tile, g.Y, row0, g.M);
     }
@@ -472,7 +477,8 @@ int assist_mlp_predict(struct assist_mlp* n, const float* d_X, int M, float* d_Y
     g.X = d_X;
     g.Y = d_Y;
     g.M = M;
-    // one workgroup per CU (the staged weights take half of its LDS), or fewer for small batches
+    // one workgroup per CU (the staged weights and 16 activation tiles fill its LDS), or fewer
+    // for small batches
     const int tiles = (M + 15) / 16;
     int blocks = (tiles + kPWaves - 1) / kPWaves;
     if (blocks > ncu) blocks = ncu;
