@@ -6,7 +6,8 @@ is outside the bar *and* its own measured trajectory comes within 3 Hill radii, 
 printed, and they must stay under 5% of the configuration.
 
 * Configuration 2 in full (4 096 main-belt systems x (1 + 6 variational), 3 650 days) against
-  the oracle, system by system: <= 1 m in position, variational states to 1e-6.
+  the oracle, system by system: <= 1 m in position, variational states to 1e-6; and the mixed form
+  with the learned start against the same oracle run, <= 1 m in position.
 * Configuration 4 in full (262 144 systems x (1 + 6 variational), 365 days, 70/30 main-belt/NEO,
   seed 4), through size-independent properties:
   - two FP64 launches are bit-identical (the work queue hands systems out in a different order);
@@ -45,6 +46,23 @@ def test_configuration_2_full_size_against_oracle(gpu, ephem, oracle, model):
                                   states, None)
     assert np.percentile(dpos, 99) <= 0.1
     assert dvar.max() <= 1e-6
+    # the mixed form with the learned start against the same FP64 oracle: the benchmark's own
+    # accuracy statement (<= 1 m after 10 years), with measured exemptions
+    import assist_amd.learned as L
+    net = L.StepModel.load(L.default_model_path(), max_batch=65536)
+    m, mst, _, mev = _run_device(ephem, states, np.zeros(n), np.full(n, 3650.0),
+                                 batch.BatchConfig.default(precision=batch.PRECISION_MIXED), learned=net)
+    assert (mst == 0).all()
+    mpos = np.linalg.norm(m[:, 0, :3] - o["states"][:, 0, :3], axis=-1) * AU_M
+    mvar = rel_err(m[:, 1:], o["states"][:, 1:])
+    print("configuration 2, mixed+learned against the oracle: p99 |dr| %.3e m, variational max rel %.3e, p99 %.3e; "
+          "%d evaluations against the oracle's %d" % (np.percentile(mpos, 99), mvar.max(), np.percentile(mvar, 99),
+                                                      mev.sum(), o["evals"].sum()))
+    bar_with_measured_exemptions(ephem, "configuration 2 in full, mixed+learned vs oracle", mpos, np.zeros(n),
+                                 np.full(n, 3650.0), states, None)
+    assert np.percentile(mpos, 99) <= 0.1
+    # (the mixed form evaluates the variational partial derivatives in FP32: measured 8e-5)
+    assert mvar.max() <= 5e-4
 
 
 @pytest.fixture(scope="module")
