@@ -570,7 +570,10 @@ int run_ias15(IntegrateArgs a, bool mixed, const LaunchShape& sh, int blocks_per
     if (blocks_per_cu_cap > 0 && blocks_per_cu_cap < per_cu) per_cu = blocks_per_cu_cap;
     const long long need_blocks = ((long long)a.nsys * sh.G + sh.threads - 1) / sh.threads;
     const long long max_blocks = (long long)per_cu * ncu;
-    const int blocks = (int)(need_blocks < max_blocks ? need_blocks : max_blocks);
+    // 8-lane groups fill the grid even for batches smaller than it: the kernel spreads the systems
+    // over every wave, and the groups without one take Chebyshev items of their wave
+    const bool spread = ias15_cache_words(sh.G, ext_model(a.fo)) > 0;
+    const int blocks = (int)((need_blocks < max_blocks && !spread) ? need_blocks : max_blocks);
     const int total_groups = blocks * sh.threads / sh.G;
     // scratch: the b/e records of every lane (42 doubles), then the substep ephemeris cache
     const int cw = use_cache ? ias15_cache_words(sh.G, ext_model(a.fo)) : 0;

@@ -184,9 +184,17 @@ ias15_kernel(IntegrateArgs args) {
     int n = 1, it = 0, last = 0, status = 0, attempts = 0;
     long long steps = 0, evals = 0, rejected = 0;
     bool ng = false;
-    // the queue hands out positions q; order[q] (when given) is the system at that position
-    bool active = grp < ka->nsys;
-    int sys = active ? (ka->order ? ka->order[grp] : grp) : 0;
+    // the queue hands out positions q; order[q] (when given) is the system at that position.
+    // 8-lane groups take their first positions slot-major over the grid's waves (group k of every
+    // wave, then group k + 1): a batch smaller than the grid spreads over every wave, and the
+    // groups left without a system help their wave's Chebyshev rounds from the first trip
+    int q0 = grp;
+    if constexpr (ias15_cache_words(G, Ext) > 0) {
+        const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+        q0 = (grp & 7) * nwaves + (grp >> 3);
+    }
+    bool active = q0 < ka->nsys;
+    int sys = active ? (ka->order ? ka->order[q0] : q0) : 0;
     int cur = 0, cend = 0;  // dense output: next and end index of this system's output times
     int nrec = 0;           // step archive: snapshots this system has taken
     constexpr bool kWaveRounds = ias15_cache_words(G, Ext) > 0;  // 8-lane groups
