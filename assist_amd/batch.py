@@ -41,7 +41,8 @@ class BatchConfig(C.Structure):
                 ("max_steps", C.c_longlong), ("precision", C.c_int), ("lanes_per_system", C.c_int),
                 ("block_size", C.c_int), ("blocks_per_cu", C.c_int), ("waves_per_simd", C.c_int),
                 ("use_graph", C.c_int), ("substep_cache_off", C.c_int), ("reserved_", C.c_int),
-                ("corrector_tolerance", C.c_double), ("first_pass_tolerance", C.c_double)]
+                ("corrector_tolerance", C.c_double), ("first_pass_tolerance", C.c_double),
+                ("first_pass_accel", C.c_double)]
 
     @classmethod
     def default(cls, **kw):

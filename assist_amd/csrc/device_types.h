@@ -103,8 +103,10 @@ struct IntegratorOpts {
     // system below 1e-16; > 0 = the real particle's max|db6| / max|a| below pc_tol
     double pc_tol;
     // > 0: an attempt also ends after its first pass when that pass's change (the same measure)
-    // is below pc_tol1
+    // is below pc_tol1, multiplied by max(0.1, pc_aref / max|a|) where max|a| (the same
+    // measure's denominator) exceeds pc_aref > 0
     double pc_tol1;
+    double pc_aref;
 };
 
 enum SystemStatus : int {

@@ -512,9 +512,14 @@ ias15_kernel(IntegrateArgs args) {
         // IAS15's test takes every vector of the system to FP64 round-off; with a tolerance the
         // real particle's own change decides (the variational vectors take the same iterations),
         // and with a first-pass tolerance an attempt whose first pass changed b6 by less ends there
+        // (the tolerance shrinks by pc_aref / max|a|, down to a tenth, where the acceleration
+        // exceeds pc_aref)
         const double pc_tol = ka->io.pc_tol;
-        const double pc_err = pc_tol > 0.0 ? gbcast<G, 0>(maxdb6 / maxak) : gmax<G>(maxdb6) / gmax<G>(maxak);
-        if (!(pc_err < (pc_tol > 0.0 ? pc_tol : 1e-16)) && !(it == 1 && pc_err < ka->io.pc_tol1) &&
+        const double pc_mak = pc_tol > 0.0 ? gbcast<G, 0>(maxak) : gmax<G>(maxak);
+        const double pc_err = pc_tol > 0.0 ? gbcast<G, 0>(maxdb6 / maxak) : gmax<G>(maxdb6) / pc_mak;
+        const double pc_aref = ka->io.pc_aref;
+        const double pc_tol1 = (pc_aref > 0.0 && pc_mak > pc_aref) ? ka->io.pc_tol1 * fmax(0.1, pc_aref / pc_mak) : ka->io.pc_tol1;
+        if (!(pc_err < (pc_tol > 0.0 ? pc_tol : 1e-16)) && !(it == 1 && pc_err < pc_tol1) &&
             !(it > 2 && pc_err_last <= pc_err) && it < 12) {
             pc_err_last = pc_err;
             it++;

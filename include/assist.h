@@ -282,9 +282,16 @@ struct assist_batch_config {
                              real particle, in either precision.  < 0: IAS15's own test. */
     double first_pass_tolerance; /* a step attempt also ends after its first predictor-corrector
                              pass when that pass changed the real particle's b6 by less than
-                             this fraction of its acceleration (the same measure).  0 (default):
-                             by precision -- the mixed form 0.3 epsilon, FP64 none.  > 0: that
-                             tolerance, in either precision.  < 0: none. */
+                             this fraction of its acceleration (the same measure), tightened by
+                             first_pass_accel below.  0 (default): by precision -- the mixed form
+                             3 epsilon, FP64 none.  > 0: that tolerance, in either precision.
+                             < 0: none. */
+    double first_pass_accel; /* acceleration scale of the first-pass test (au/day^2): where the
+                             real particle's max|a| exceeds it, the first-pass tolerance is
+                             multiplied by first_pass_accel / max|a|, down to a tenth -- close
+                             to the Sun and near planets the test tightens.  0 (default): 6e-5
+                             (the Sun's pull at about 2.2 au).  > 0: that scale.  < 0: none (a
+                             purely relative test). */
 };
 
 struct assist_batch_stats {

@@ -10,7 +10,8 @@
  *    (at most 12; stop early when the error stops decreasing after 3); as an
  *    option of the product (cfg->pc_tol > 0), until the real particles' own
  *    max|delta b6| / max|a| is below pc_tol, and (cfg->pc_tol1 > 0) after the
- *    first pass of an attempt when that pass changed them by less than pc_tol1,
+ *    first pass of an attempt when that pass changed them by less than pc_tol1
+ *    (times max(0.1, pc_aref / max|a|) where max|a| exceeds cfg->pc_aref > 0),
  *  - the global step-size criterion dt' = dt (eps / (max|b6| / max|a|))^(1/7)
  *    over the real particles (ASSIST selects this "legacy global" mode,
  *    reference src/assist.c:446), rejection when dt' < dt/4, growth <= 4x,
@@ -118,6 +119,7 @@ void orc_ias15_cfg_default(orc_ias15_cfg *c) {
     c->max_steps = 100000000L;
     c->pc_tol = 0.0;
     c->pc_tol1 = 0.0;
+    c->pc_aref = 0.0;
 }
 
 static inline void add_cs(double *p, double *cs, double in) {
@@ -237,13 +239,16 @@ int orc_integrate(const orc_ephem *eph, const orc_opts *o, orc_ias15_cfg *cfg, o
                 B.g[5][k] = b6 * D[20] + b5;
                 B.g[6][k] = b6;
             }
-            double pc_err = 1e300, pc_err_last = 2.0;
+            double pc_err = 1e300, pc_err_last = 2.0, pc_mak = 0.0;
             int it = 0;
             const double pc_tol = cfg->pc_tol > 0 ? cfg->pc_tol : 1e-16;
             const int pc_n = cfg->pc_tol > 0 ? n3r : n3; /* components in the test */
             for (;;) {
                 if (pc_err < pc_tol) break;
-                if (it == 1 && cfg->pc_tol1 > 0 && pc_err < cfg->pc_tol1) break;
+                if (it == 1 && cfg->pc_tol1 > 0) {
+                    const double tol1 = (cfg->pc_aref > 0 && pc_mak > cfg->pc_aref) ? cfg->pc_tol1 * fmax(0.1, cfg->pc_aref / pc_mak) : cfg->pc_tol1;
+                    if (pc_err < tol1) break;
+                }
                 if (it > 2 && pc_err_last <= pc_err) break;
                 if (it >= 12) { st->pc_maxed++; break; }
                 pc_err_last = pc_err;
@@ -366,6 +371,7 @@ int orc_integrate(const orc_ephem *eph, const orc_opts *o, orc_ias15_cfg *cfg, o
                 if (s->status) break;
                 st->iterations++;
                 pc_err = maxdb6 / maxak;
+                pc_mak = maxak;
             }
             if (s->status) break;
             /* step-size control (global criterion over real particles) */

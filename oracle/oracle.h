@@ -94,6 +94,10 @@ typedef struct {
     /* > 0: also stop after the first pass of an attempt when its change (the same measure) is
      * below pc_tol1 (the product's first-pass option); 0 = no first-pass stop */
     double pc_tol1;
+    /* > 0: where the pass's max|a| (the measure's denominator) exceeds pc_aref, the first-pass
+     * tolerance is pc_tol1 * max(0.1, pc_aref / max|a|) (the product's first-pass acceleration
+     * scale) */
+    double pc_aref;
 } orc_ias15_cfg;
 
 typedef struct {

@@ -25,7 +25,7 @@ class OrcOpts(C.Structure):
 class OrcIas15Cfg(C.Structure):
     _fields_ = [("epsilon", C.c_double), ("min_dt", C.c_double), ("dt", C.c_double),
                 ("exact_finish", C.c_int), ("max_steps", C.c_long), ("pc_tol", C.c_double),
-                ("pc_tol1", C.c_double)]
+                ("pc_tol1", C.c_double), ("pc_aref", C.c_double)]
 
 
 class OrcEphem(C.Structure):

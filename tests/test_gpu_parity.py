@@ -373,12 +373,15 @@ def test_corrector_tolerance_against_oracle(gpu, ephem, oracle, model):
 
 
 
-def test_first_pass_tolerance_against_oracle(gpu, ephem, oracle, model):
+@pytest.mark.parametrize("accel", [0.0, -1.0])
+def test_first_pass_tolerance_against_oracle(gpu, ephem, oracle, model, accel):
     """The mixed form's first-pass stop (an attempt ends after its first predictor-corrector pass
-    when that pass changed the real particle's b6 by less than 0.3 epsilon of its acceleration)
-    run in FP64 against the oracle with the same options, on the configuration-4 mix of
-    main-belt and NEO-like orbits x (1 + 6 variational), 10 years: within 1 m and the same step
-    counts to within 2; and its own effect against IAS15's standard stop."""
+    when that pass changed the real particle's b6 by less than 3 epsilon of its acceleration, times
+    max(0.1, 6e-5 au/day^2 / max|a|) where max|a| exceeds 6e-5; accel = -1: no scale) run in FP64
+    against the oracle with the same
+    options, on the configuration-4 mix of main-belt and NEO-like orbits x (1 + 6 variational),
+    10 years: within 1 m and the same step counts to within 2; and its own effect against IAS15's
+    standard stop."""
     from assist_amd import batch, synthetic
     rng = np.random.default_rng(72)
     n = 64
@@ -386,9 +389,11 @@ def test_first_pass_tolerance_against_oracle(gpu, ephem, oracle, model):
     states[:, 0] = synthetic.mixed_population(model, n, rng, 0.3)[0]
     states[:, 1:] = synthetic.variational_identity(n, 6)
     span = 3652.5
-    cfg = batch.BatchConfig.default(corrector_tolerance=1e-11, first_pass_tolerance=3e-10, min_dt=0.001)
+    cfg = batch.BatchConfig.default(corrector_tolerance=1e-11, first_pass_tolerance=3e-9, first_pass_accel=accel,
+                                    min_dt=0.001)
     r = batch.integrate(ephem, 0.0, span, states, config=cfg)
-    o = oracle.integrate(0.0, span, states, cfg=oracle.cfg(pc_tol=1e-11, pc_tol1=3e-10, min_dt=0.001))
+    o = oracle.integrate(0.0, span, states, cfg=oracle.cfg(pc_tol=1e-11, pc_tol1=3e-9, pc_aref=6e-5 if accel == 0 else 0.0,
+                                                           min_dt=0.001))
     std = batch.integrate(ephem, 0.0, span, states, config=batch.BatchConfig.default(min_dt=0.001))
     tol = batch.integrate(ephem, 0.0, span, states, config=batch.BatchConfig.default(corrector_tolerance=1e-11,
                                                                                        min_dt=0.001))
@@ -396,11 +401,13 @@ def test_first_pass_tolerance_against_oracle(gpu, ephem, oracle, model):
     d = np.linalg.norm(r.states[:, 0, :3] - o["states"][:, 0, :3], axis=-1) * AU_M
     e = np.linalg.norm(r.states[:, 0, :3] - std.states[:, 0, :3], axis=-1) * AU_M
     dv = np.abs(r.states[:, 1:] - o["states"][:, 1:]).max() / np.abs(o["states"][:, 1:]).max()
-    print("first-pass tolerance 3e-10, 10 yr: vs oracle max %.3e m (variational %.1e); vs the standard stop "
-          "max %.3e m; evaluations %d vs %d standard (%.3fx), %d with the corrector tolerance alone"
-          % (d.max(), dv, e.max(), r.stats.evals, std.stats.evals, std.stats.evals / r.stats.evals, tol.stats.evals))
+    print("first-pass tolerance 3e-9, accel %g, 10 yr: vs oracle max %.3e m (variational %.1e); vs the standard "
+          "stop max %.3e m; evaluations %d vs %d standard (%.3fx; oracle %d), %d with the corrector tolerance alone"
+          % (accel, d.max(), dv, e.max(), r.stats.evals, std.stats.evals, std.stats.evals / r.stats.evals,
+             o["evals"].sum(), tol.stats.evals))
     assert d.max() <= 1.0
     assert dv <= 1e-6
     assert np.abs(r.steps - o["steps"]).max() <= 2
+    assert abs(r.stats.evals / o["evals"].sum() - 1) < 0.01
     assert e.max() <= 1.0
     assert r.stats.evals < tol.stats.evals
