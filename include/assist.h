@@ -270,9 +270,12 @@ struct assist_batch_config {
     int use_graph;        /* 1: launch the integration as a captured HIP graph, captured once per
                              argument set and replayed (the launches of a replay take the same
                              device pointers); 0 (default): direct launches */
-    int substep_cache_off; /* 0 (default): 8-lane groups keep the ephemeris positions of each
-                             substep for the later predictor-corrector iterations of the step
-                             (results are the same bit for bit); 1: evaluate them every time */
+    int substep_cache_off; /* whether 8-lane groups keep the ephemeris positions of each substep
+                             for the later predictor-corrector iterations of the step (results
+                             are the same bit for bit either way).  0 (default): keep them unless
+                             the first-pass stop is on (the mixed form's default), where most
+                             attempts take one pass and the stores cost more than the reuse
+                             saves; 1: evaluate them every time; -1: keep them */
     int reserved_;
     double corrector_tolerance; /* when the predictor-corrector iterations stop.  0 (default): by
                              precision -- FP64 takes IAS15's own test (every vector's

@@ -4,8 +4,8 @@ The first predictor-corrector iteration of a step attempt stores the raw Chebysh
 its 7 substeps, the later iterations load them (the reference's per-body cache keyed by the exact
 time, reference src/forces.c:175-263, serves the same repeated substep times), and the groups
 that need fresh positions share the Chebyshev rounds of their wave.  Each case runs the same
-batch with the cache on and off (assist_batch_config.substep_cache_off: every trip then
-evaluates every group's positions in the wave-wide rounds) and compares states, status, step
+batch with the cache on and off (assist_batch_config.substep_cache_off = -1 and 1: every trip
+then evaluates every group's positions in the wave-wide rounds) and compares states, status, step
 counts and the launch's evaluation and rejection totals exactly.  (Agreement with the oracle,
 with the cache on, is the subject of the parity tests.)
 """
@@ -24,7 +24,8 @@ SHAPES = [(8, 6), (8, 0)]
 
 def _pair(ephem, t0, t1, states, params, lanes, **cfg):
     from assist_amd import batch
-    on = batch.integrate(ephem, t0, t1, states, params, config=batch.BatchConfig.default(lanes_per_system=lanes, **cfg))
+    on = batch.integrate(ephem, t0, t1, states, params,
+                         config=batch.BatchConfig.default(lanes_per_system=lanes, substep_cache_off=-1, **cfg))
     off = batch.integrate(ephem, t0, t1, states, params,
                           config=batch.BatchConfig.default(lanes_per_system=lanes, substep_cache_off=1, **cfg))
     return on, off
@@ -80,7 +81,7 @@ def test_cache_dense_output(gpu, ephem, model, lanes, n_var):
     rng = np.random.default_rng(44)
     outs = [np.sort(rng.uniform(0.0, 90.0, 5)) for _ in range(n)]
     on = batch.integrate_dense(ephem, np.zeros(n), outs, states,
-                               config=batch.BatchConfig.default(lanes_per_system=lanes))
+                               config=batch.BatchConfig.default(lanes_per_system=lanes, substep_cache_off=-1))
     off = batch.integrate_dense(ephem, np.zeros(n), outs, states,
                                 config=batch.BatchConfig.default(lanes_per_system=lanes, substep_cache_off=1))
     assert (on.status == 0).all()
