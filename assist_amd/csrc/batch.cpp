@@ -618,20 +618,22 @@ int run_ias15(IntegrateArgs a, bool mixed, const LaunchShape& sh, int blocks_per
 }
 
 struct TuneKey {
-    int dev, nvec, mixed, forces, has_params;
+    int dev, nvec, mixed, forces, has_params, cache;
     bool operator<(const TuneKey& o) const {
-        return std::tie(dev, nvec, mixed, forces, has_params) < std::tie(o.dev, o.nvec, o.mixed, o.forces, o.has_params);
+        return std::tie(dev, nvec, mixed, forces, has_params, cache) <
+               std::tie(o.dev, o.nvec, o.mixed, o.forces, o.has_params, o.cache);
     }
 };
 std::map<TuneKey, LaunchShape> g_tuned;
 
 // Picks lanes per system, block size and kernel variant for this batch's configuration by
 // timing each candidate on copies of the first kTuneSample systems over at most kTuneSpan days
-// (the caller's buffers are not touched); the choice is cached per device and configuration.
-int tuned_shape(const IntegrateArgs& base, bool mixed, hipStream_t s, LaunchShape* out) {
+// (the caller's buffers are not touched), with the substep cache as the launch will run it; the
+// choice is cached per device and configuration.
+int tuned_shape(const IntegrateArgs& base, bool mixed, bool use_cache, hipStream_t s, LaunchShape* out) {
     int dev = 0;
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return ASSIST_BATCH_ERROR_DEVICE;
-    const TuneKey key{dev, base.nvec, mixed ? 1 : 0, base.fo.forces, base.params ? 1 : 0};
+    const TuneKey key{dev, base.nvec, mixed ? 1 : 0, base.fo.forces, base.params ? 1 : 0, use_cache ? 1 : 0};
     auto it = g_tuned.find(key);
     if (it != g_tuned.end()) {
         *out = it->second;
@@ -688,7 +690,7 @@ int tuned_shape(const IntegrateArgs& base, bool mixed, hipStream_t s, LaunchShap
                             "hipMemcpyAsync"))
                     return ASSIST_BATCH_ERROR_DEVICE;
                 float t_ms = 0.f;
-                const int rc = run_ias15(a, mixed, sh, 0, s, &t_ms, nullptr);
+                const int rc = run_ias15(a, mixed, sh, 0, s, &t_ms, nullptr, false, use_cache);
                 if (rc) return rc;
                 if (t_ms < ms) ms = t_ms;
             }
@@ -946,8 +948,11 @@ static int integrate_device_locked(const struct assist_ephem* ephem, const struc
     if (ext && (mixed || shape.waves != 1)) return ASSIST_BATCH_ERROR_UNSUPPORTED;
     const bool tune = lanes_req == 0 && cfg.block_size == 0 && cfg.waves_per_simd == 0 &&
                       n >= kTuneMinSystems && !dense && !ext;
+    // the substep cache pays when attempts take several passes: by default it is off with the
+    // first-pass stop (configuration 4 mixed+learned: 158.3 ms without it, 161.9 ms with it)
+    const bool cache = cfg.substep_cache_off < 0 || (cfg.substep_cache_off == 0 && a.io.pc_tol1 <= 0.0);
     if (tune) {
-        rc = tuned_shape(a, mixed, s, &shape);
+        rc = tuned_shape(a, mixed, cache, s, &shape);
         if (rc) return rc;
     }
     // large batches: hand out the systems of highest expected cost first (longest spans, smallest
@@ -965,9 +970,6 @@ static int integrate_device_locked(const struct assist_ephem* ephem, const struc
     }
     float ms = 0.f;
     int blocks = 0;
-    // the substep cache pays when attempts take several passes: by default it is off with the
-    // first-pass stop (configuration 4 mixed+learned: 158.3 ms without it, 161.9 ms with it)
-    const bool cache = cfg.substep_cache_off < 0 || (cfg.substep_cache_off == 0 && a.io.pc_tol1 <= 0.0);
     rc = run_ias15(a, mixed, shape, cfg.blocks_per_cu, s, &ms, &blocks, cfg.use_graph != 0, cache);
     if (rc) return rc;
     const int G = shape.G, threads = shape.threads, waves = shape.waves;
