@@ -55,8 +55,8 @@ bool hip_ok(hipError_t e, const char* what) {
     return false;
 }
 
-// Flattens one SPK target into records [jd_ref-jul(MID), 86400/RADIUS, x, y, z coefficients],
-// each coordinate zero-padded to kCoefPad coefficients.
+// Flattens one SPK target into records [jd_ref-jul(MID), 86400/RADIUS, then the coefficients
+// interleaved by degree: x_0, y_0, z_0, x_1, ...], each coordinate zero-padded to kCoefPad.
 // Returns false when the target's segments are not contiguous with equal record length.
 bool flatten_target(const spk_s* s, const spk_target& t, double jd_ref, std::vector<double>& out, DevSlot& slot,
                     size_t& offset_words) {
@@ -87,7 +87,8 @@ bool flatten_target(const spk_s* s, const spk_target& t, double jd_ref, std::vec
             const double* rec = src + (size_t)r * rsize;
             dst[0] = jd_ref - jul(rec[0]);
             dst[1] = 1.0 / (rec[1] / 86400.0);
-            for (int c = 0; c < 3; c++) memcpy(dst + 2 + c * kCoefPad, rec + 2 + c * P, sizeof(double) * P);
+            for (int c = 0; c < 3; c++)
+                for (int p = 0; p < P; p++) dst[2 + 3 * p + c] = rec[2 + c * P + p];
             dst += kRecWords;
         }
     }
@@ -100,8 +101,8 @@ bool flatten_target(const spk_s* s, const spk_target& t, double jd_ref, std::vec
 }
 
 // Re-lays one column of a JPL binary planets file as device records, one per sub-interval of
-// `inc / niv` days: [jd_ref - JD(mid), 1/radius (1/day), x, y, z coefficients * scale], each
-// coordinate zero-padded to kCoefPad.  The evaluation is then the SPK one: the record index
+// `inc / niv` days: [jd_ref - JD(mid), 1/radius (1/day), then the coefficients * scale
+// interleaved by degree as above], each coordinate zero-padded to kCoefPad.  The evaluation is then the SPK one: the record index
 // (jd_ref - beg + t) niv / inc and z = (jd_ref - mid + t) / radius are the block and sub-interval
 // selection of reference src/ascii_ephem.c:315-318 and :34-39, written over the whole file.
 // `scale` folds a constant factor into the coefficients (the Earth's and the Moon's offsets from
@@ -125,7 +126,7 @@ bool flatten_ascii_column(const ascii_s* a, int col, double scale, double jd_ref
             dst[0] = jd_ref - (a->beg + (double)blk * a->inc + (k + 0.5) * sub);
             dst[1] = 2.0 / sub;
             for (int m = 0; m < 3; m++)
-                for (int p = 0; p < ncf; p++) dst[2 + m * kCoefPad + p] = cf[m * ncf + p] * scale;
+                for (int p = 0; p < ncf; p++) dst[2 + 3 * p + m] = cf[m * ncf + p] * scale;
             dst += kRecWords;
         }
     }

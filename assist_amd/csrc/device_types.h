@@ -3,7 +3,8 @@
 // The SPK type-2 data of the 28 Chebyshev targets the force model needs are
 // flattened into one device buffer.  Each target ("slot") is a contiguous run
 // of equal-length records; a record holds
-//     [ jd_ref - jul(MID),  86400/RADIUS,  x coeffs, y coeffs, z coeffs ]
+//     [ jd_ref - jul(MID),  86400/RADIUS,  x_0, y_0, z_0,  x_1, y_1, z_1,  ... ]
+// (the coefficients interleaved by degree, in the order the evaluation uses them)
 // with every coordinate zero-padded to kCoefPad coefficients, so that all
 // records have one size and one fully unrolled evaluation (a zero coefficient
 // adds an exact zero).

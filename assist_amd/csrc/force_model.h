@@ -205,17 +205,19 @@ AHD ChebOut<D> cheb_series(const DevSlot& S, Ptr rec, D t) {
     }
     const int P = std::is_same<D, double>::value ? kCoefPad : S.P;
     const D z = (D(c[0]) + t) * D(c[1]);
+    // coefficient p of x, y and z at words 2 + 3p, 3 + 3p, 4 + 3p: the series consumes the
+    // record in load order, so its first terms start while the later loads are in flight
     const double* cx = c + 2;
-    const double* cy = cx + kCoefPad;
-    const double* cz = cy + kCoefPad;
+    const double* cy = c + 3;
+    const double* cz = c + 4;
     const D z2 = D(2.0) * z;
     D T0 = D(1.0), T1 = z;
-    D x = fma(D(cx[1]), z, D(cx[0]));
-    D y = fma(D(cy[1]), z, D(cy[0]));
-    D w = fma(D(cz[1]), z, D(cz[0]));
+    D x = fma(D(cx[3]), z, D(cx[0]));
+    D y = fma(D(cy[3]), z, D(cy[0]));
+    D w = fma(D(cz[3]), z, D(cz[0]));
     ChebOut<D> o;
     D S0 = D(0.0), S1 = D(1.0);
-    D vx = D(cx[1]), vy = D(cy[1]), vz = D(cz[1]);
+    D vx = D(cx[3]), vy = D(cy[3]), vz = D(cz[3]);
     // second derivatives U_p = d^2 T_p / dz^2: U_0 = U_1 = 0 (reference src/spk.c:520-530)
     D U0 = D(0.0), U1 = D(0.0);
     D ax = D(0.0), ay = D(0.0), az = D(0.0);
@@ -223,19 +225,19 @@ AHD ChebOut<D> cheb_series(const DevSlot& S, Ptr rec, D t) {
     for (int p = 2; p < kCoefPad; p++) {
         if (p < P) {
             const D T2 = fma(z2, T1, -T0);
-            x = fma(D(cx[p]), T2, x);
-            y = fma(D(cy[p]), T2, y);
-            w = fma(D(cz[p]), T2, w);
+            x = fma(D(cx[3 * p]), T2, x);
+            y = fma(D(cy[3 * p]), T2, y);
+            w = fma(D(cz[3 * p]), T2, w);
             if (Vel || Acc) {
                 const D S2 = fma(z2, S1, fma(D(2.0), T1, -S0));
-                vx = fma(D(cx[p]), S2, vx);
-                vy = fma(D(cy[p]), S2, vy);
-                vz = fma(D(cz[p]), S2, vz);
+                vx = fma(D(cx[3 * p]), S2, vx);
+                vy = fma(D(cy[3 * p]), S2, vy);
+                vz = fma(D(cz[3 * p]), S2, vz);
                 if (Acc) {
                     const D U2 = fma(z2, U1, fma(D(4.0), S1, -U0));
-                    ax = fma(D(cx[p]), U2, ax);
-                    ay = fma(D(cy[p]), U2, ay);
-                    az = fma(D(cz[p]), U2, az);
+                    ax = fma(D(cx[3 * p]), U2, ax);
+                    ay = fma(D(cy[3 * p]), U2, ay);
+                    az = fma(D(cz[3 * p]), U2, az);
                     U0 = U1;
                     U1 = U2;
                 }
