@@ -17,8 +17,10 @@ __host__ __device__ constexpr int ias15_cols_per_group(int G, int nvec, int wave
     return (waves == 2 && G == 8 && nvec <= 7) ? 7 : G;
 }
 // IAS15 tables staged in LDS ahead of the columns (doubles): spacings h (8), predictor weights
-// (9 x 8 and 9 x 7), 1/(h_i - h_j) (28), Newton-to-power weights (21).
-constexpr int kTabH = 0, kTabPX = 8, kTabPV = 80, kTabRRI = 143, kTabC = 171, kTabWords = 192;
+// (9 x 8 and 9 x 7), 1/(h_i - h_j) (28), Newton-to-power weights (21), and the b-update weights
+// of substep n as a full row (7 x 7: the weights, then 1 at k = n - 1 and 0 after it), padded
+// to 256 words so that the columns keep their LDS bank alignment.
+constexpr int kTabH = 0, kTabPX = 8, kTabPV = 80, kTabRRI = 143, kTabC = 171, kTabW = 192, kTabWords = 256;
 
 struct ForcesArgs {
     const DevEphem* E;

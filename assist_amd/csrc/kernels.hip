@@ -156,7 +156,13 @@ ias15_kernel(IntegrateArgs args) {
         else if (i < kTabPV) v = cPX[i - kTabPX];
         else if (i < kTabRRI) v = cPV[i - kTabPV];
         else if (i < kTabC) v = cRRI[i - kTabRRI];
-        else v = cC[i - kTabC];
+        else if (i < kTabW) v = cC[i - kTabC];
+        else if (i < kTabW + 49) {
+            const int n = (i - kTabW) / 7 + 1, k = (i - kTabW) % 7;
+            v = k < n - 1 ? cC[(n - 1) * (n - 2) / 2 + k] : (k == n - 1 ? 1.0 : 0.0);
+        } else {
+            v = 0.0;  // (padding)
+        }
         tab[i] = v;
     }
     // the ephemeris slot descriptors, read by every force evaluation with a lane-dependent
@@ -461,40 +467,37 @@ ias15_kernel(IntegrateArgs args) {
         }
         // ---- corrector: improve g_{n-1} and b_0..b_{n-1}.  Straight-line code for every
         //      substep (the groups of a wave are at different substeps): each row of b and g is
-        //      read from LDS at once and the substep-dependent steps are selected, not branched,
-        //      so the LDS latency is paid once per trip. ----
+        //      read from LDS at once and the substep-dependent steps are selected or weighted,
+        //      not branched, so the LDS latency is paid once per trip. ----
         double maxak = 0.0, maxdb6 = 0.0;
         {
-            const double* rri = tab + kTabRRI + n * (n - 1) / 2;      // 1 / (h_n - h_j), j = 0..n-1
-            const double* cc = tab + kTabC + (n - 1) * (n - 2) / 2;   // b_k += cc[k] dg_{n-1}, k < n-1
-            double r[7], w[6];
+            const double* rri = tab + kTabRRI + n * (n - 1) / 2;  // 1 / (h_n - h_j), j = 0..n-1
+            const double* wn = tab + kTabW + 7 * (n - 1);          // b_k += wn[k] dg_{n-1}
+            double r[6], w[7];
 #pragma unroll
-            for (int k = 0; k < 7; k++) r[k] = rri[k < n ? k : 0];
+            for (int k = 0; k < 6; k++) r[k] = rri[k < n ? k : 0];
 #pragma unroll
-            for (int k = 0; k < 6; k++) w[k] = cc[k < n - 1 ? k : 0];
-            double rl = r[0];
-#pragma unroll
-            for (int k = 1; k < 7; k++) rl = (k == n - 1) ? r[k] : rl;
+            for (int k = 0; k < 7; k++) w[k] = wn[k];
+            const double rl = rri[n - 1];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                double gv[7], bv[7];
+                double gv[6], bv[7];
 #pragma unroll
-                for (int k = 0; k < 7; k++) { gv[k] = g[k][c]; bv[k] = b[k][c]; }
+                for (int k = 0; k < 6; k++) gv[k] = g[k][c];
+#pragma unroll
+                for (int k = 0; k < 7; k++) bv[k] = b[k][c];
+                const double gold = g[n - 1][c];
                 // ((gk r_0 - g_0) r_1 - g_1) r_2 ..., each step one fused multiply-add
                 double val = at[c] - a0[c];
 #pragma unroll
                 for (int k = 0; k < 6; k++) val = (k < n - 1) ? fma(val, r[k], -gv[k]) : val;
                 val *= rl;
-                double gold = gv[0];
-#pragma unroll
-                for (int k = 1; k < 7; k++) gold = (k == n - 1) ? gv[k] : gold;
                 const double tmp = val - gold;
                 g[n - 1][c] = val;
+                // b_k + w_k tmp: the weights below k = n - 1, then b_{n-1} + tmp (weight 1), and
+                // the rows above unchanged (weight 0)
 #pragma unroll
-                for (int k = 0; k < 7; k++) {
-                    const double nb = (k < n - 1) ? fma(tmp, w[k < 6 ? k : 5], bv[k]) : ((k == n - 1) ? bv[k] + tmp : bv[k]);
-                    b[k][c] = nb;
-                }
+                for (int k = 0; k < 7; k++) b[k][c] = fma(tmp, w[k], bv[k]);
                 if (n == 7 && own) {
                     const double ak = fabs(at[c]);
                     if (is_normal(ak) && ak > maxak) maxak = ak;
