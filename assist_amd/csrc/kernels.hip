@@ -641,11 +641,16 @@ ias15_kernel(IntegrateArgs args) {
         }
         if (own) {
             const double q = (dt_last != 0.0) ? dts / dt_last : 0.0;
+            // the last step's b, loaded at once (one wait instead of one per component: the
+            // stores below would otherwise keep the later components' loads behind them)
+            double bl[21];
+#pragma unroll
+            for (int k = 0; k < 21; k++) bl[k] = scr[k];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 double br[7], e[7];
 #pragma unroll
-                for (int k = 0; k < 7; k++) br[k] = scr[3 * k + c];
+                for (int k = 0; k < 7; k++) br[k] = bl[3 * k + c];
                 if (dt_last != 0.0 && q <= 20.0) predict_e(br, q, e);
                 else
 #pragma unroll
