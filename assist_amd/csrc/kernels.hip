@@ -165,16 +165,19 @@ ias15_kernel(IntegrateArgs args) {
         }
         tab[i] = v;
     }
-    // the ephemeris slot descriptors, read by every force evaluation with a lane-dependent
-    // index, are staged in LDS as well (a vector load of them would sit in front of every
-    // record fetch)
-    __shared__ DevSlot s_slots[kNumSlots];
-    __shared__ double s_gm[kNumBodies];
+    // the ephemeris descriptor (slot descriptors, masses and force-model constants), read by
+    // every force evaluation, is staged in LDS as well: the slot descriptors are read with a
+    // lane-dependent index in front of every record fetch, and the constants' global loads would
+    // each wait on the memory latency in the middle of a trip
+    static_assert(sizeof(DevEphem) % sizeof(unsigned long long) == 0, "DevEphem is copied in 8-byte words");
+    __shared__ DevEphem s_E;
     {
-        const DevSlot* src = ka->E->slot;
-        for (int i = threadIdx.x; i < kNumSlots; i += blockDim.x) s_slots[i] = src[i];
-        for (int i = threadIdx.x; i < kNumBodies; i += blockDim.x) s_gm[i] = ka->E->GM[i];
+        const unsigned long long* src = (const unsigned long long*)ka->E;
+        unsigned long long* dst = (unsigned long long*)&s_E;
+        for (int i = threadIdx.x; i < (int)(sizeof(DevEphem) / sizeof(unsigned long long)); i += blockDim.x) dst[i] = src[i];
     }
+    const DevSlot* const s_slots = s_E.slot;
+    const double* const s_gm = s_E.GM;
     __syncthreads();
     const int cols_per_group = ias15_cols_per_group(G, nv, W);
     const int col = (own || cols_per_group == G) ? (int)(threadIdx.x / G) * cols_per_group + lane
@@ -403,7 +406,7 @@ ias15_kernel(IntegrateArgs args) {
     // and leaves the trip right after the evaluation without touching any state or output.
     while (kWaveRounds ? (__ballot(active) != 0ULL) : active) {
         ka = launder(ka);
-        const DevEphem& E = *ka->E;
+        const DevEphem& E = s_E;
         // zero-length span (dense output: no output times left)
         if (active && !phase && (ka->out_t ? cur >= cend : t == t_end)) { finish(); continue; }
         // ---- evaluation point of this trip ----
