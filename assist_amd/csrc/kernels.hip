@@ -246,11 +246,31 @@ ias15_kernel(IntegrateArgs args) {
     };
 
     auto load = [&](int s) {
+        // the system's inputs are loaded first, all together, so that they wait on memory once
         const double* st = ka->states + ((size_t)s * nv + (own ? lane : 0)) * 6;
+        double sv[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) sv[k] = st[k];
+        double pa[3] = {0.0, 0.0, 0.0}, pd[3] = {0.0, 0.0, 0.0};
+        if (ka->params) {
+            const double* pp = ka->params + (size_t)s * nv * 3;
+            const double* pl = pp + 3 * (own ? lane : 0);
+#pragma unroll
+            for (int c = 0; c < 3; c++) { pa[c] = pp[c]; pd[c] = pl[c]; }
+        }
+        // (the optional arrays are read through a pointer that falls back on t0[s], which is
+        // always there, so that no load sits behind a branch that would wait for it)
+        const double t0s = ka->t0[s];
+        const bool dense = ka->out_t != nullptr;
+        const int* po = dense ? ka->out_off + s : (const int*)(ka->t0 + s);
+        const int off0 = po[0], off1 = po[dense ? 1 : 0];
+        const double t1s = (dense ? ka->t0 : ka->t1)[s];
+        const double dtl = (ka->dt0 ? ka->dt0 : ka->t0)[s];
+        const double dt0s = ka->dt0 ? fabs(dtl) : ka->io.dt_initial;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            x0[c] = own ? st[c] : 0.0;
-            v0[c] = own ? st[3 + c] : 0.0;
+            x0[c] = own ? sv[c] : 0.0;
+            v0[c] = own ? sv[3 + c] : 0.0;
             csx[c] = csv[c] = a0[c] = 0.0;
         }
 #pragma unroll
@@ -260,20 +280,19 @@ ias15_kernel(IntegrateArgs args) {
         A[0] = A[1] = A[2] = dA[0] = dA[1] = dA[2] = 0.0;
         ng = false;
         if (ka->params) {
-            const double* pp = ka->params + (size_t)s * nv * 3;
-            A[0] = pp[0]; A[1] = pp[1]; A[2] = pp[2];
+            A[0] = pa[0]; A[1] = pa[1]; A[2] = pa[2];
             ng = !(A[0] == 0.0 && A[1] == 0.0 && A[2] == 0.0);
-            if (own) { dA[0] = pp[3 * lane]; dA[1] = pp[3 * lane + 1]; dA[2] = pp[3 * lane + 2]; }
+            if (own) { dA[0] = pd[0]; dA[1] = pd[1]; dA[2] = pd[2]; }
         }
-        t = ka->t0[s];
+        t = t0s;
         if (ka->out_t) {
-            cur = ka->out_off[s];
-            cend = ka->out_off[s + 1];
+            cur = off0;
+            cend = off1;
             t_end = (cend > cur) ? ka->out_t[cend - 1] : t;
         } else {
-            t_end = ka->t1[s];
+            t_end = t1s;
         }
-        dt = ka->dt0 ? fabs(ka->dt0[s]) : ka->io.dt_initial;
+        dt = dt0s;
         if (dt * (t_end - t) < 0) dt = -dt;
         dt_last = 0.0;
         phase = false;
