@@ -27,6 +27,10 @@ constexpr int kNumBodies = 27;      // ASSIST body indices 0..26
 constexpr int kNumPlanets = 11;     // ASSIST_BODY_NPLANETS
 constexpr int kCoefPad = 14;                 // coefficients per coordinate on the device
 constexpr int kRecWords = 2 + 3 * kCoefPad;  // 44 doubles = 352 bytes per record
+// Short form of the series for records of at most kCoefShort coefficients (the trailing padded
+// terms, which add exact zeros, are skipped; 11 is the asteroid records' count of the ephemeris
+// files used here, and DE440's outer planets have 6-8)
+constexpr int kCoefShort = 11;
 constexpr int kSlotEMB = 3;         // Earth-Moon barycentre (no force of its own)
 constexpr int kSlotEarth = 4;       // Earth relative to the EMB
 constexpr int kSlotMoon = 5;        // Moon relative to the EMB
@@ -58,7 +62,7 @@ struct DevEphem {
     double t_beg, t_end;    // coverage of every planet-file target, days relative to jd_ref
     double ta_beg, ta_end;  // coverage of every asteroid target (the planet window when none)
     int n_ast;              // asteroids present (0 if no asteroid file)
-    int pad_;
+    int short_from;         // first slot from which every slot has at most kCoefShort coefficients
     // force-model constants (reference src/spk.c:769-789)
     double J2E, J3E, J4E, J2SUN, Re_eq, Rs_eq, c_squared, over_c_squared;
     // pole rotations: Earth (RA 0, Dec 90 deg) and Sun (RA 286.13, Dec 63.87 deg),

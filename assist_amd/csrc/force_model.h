@@ -191,19 +191,22 @@ struct ChebOut {
 
 struct alignas(16) Pair { double x, y; };
 
-// The device evaluates all kCoefPad terms of the zero-padded record with every load issued
-// up front; the operation-count instantiation (D != double) counts the file's P terms.  `rec`
+// The device evaluates the first NP terms of the zero-padded record (NP = kCoefPad, or
+// kCoefShort where every record of the slot has at most that many) with every load issued up
+// front; the operation-count instantiation (D != double) counts the file's P terms.  `rec`
 // points at the record (in global memory on the device, in host memory otherwise).
-template <typename D, bool Vel, bool Acc, typename Ptr>
+template <typename D, bool Vel, bool Acc, int NP, typename Ptr>
 AHD ChebOut<D> cheb_series(const DevSlot& S, Ptr rec, D t) {
+    static_assert(NP >= 2 && NP <= kCoefPad, "series length");
+    constexpr int NW = 2 + 3 * NP;  // words used (an odd count loads one more, inside the record)
     double c[kRecWords];
 #pragma unroll
-    for (int k = 0; k < kRecWords; k += 2) {
+    for (int k = 0; k < NW; k += 2) {
         const Pair v = {rec[k], rec[k + 1]};
         c[k] = v.x;
         c[k + 1] = v.y;
     }
-    const int P = std::is_same<D, double>::value ? kCoefPad : S.P;
+    const int P = std::is_same<D, double>::value ? NP : S.P;
     const D z = (D(c[0]) + t) * D(c[1]);
     // coefficient p of x, y and z at words 2 + 3p, 3 + 3p, 4 + 3p: the series consumes the
     // record in load order, so its first terms start while the later loads are in flight
@@ -222,7 +225,7 @@ AHD ChebOut<D> cheb_series(const DevSlot& S, Ptr rec, D t) {
     D U0 = D(0.0), U1 = D(0.0);
     D ax = D(0.0), ay = D(0.0), az = D(0.0);
 #pragma unroll
-    for (int p = 2; p < kCoefPad; p++) {
+    for (int p = 2; p < NP; p++) {
         if (p < P) {
             const D T2 = fma(z2, T1, -T0);
             x = fma(D(cx[3 * p]), T2, x);
@@ -277,7 +280,7 @@ AHD int record_index(const DevSlot& S, D t) {
     return idx < 0 ? 0 : (idx >= S.nrec ? S.nrec - 1 : idx);
 }
 
-template <typename D, bool Vel, bool Acc = false>
+template <typename D, bool Vel, bool Acc = false, int NP = kCoefPad>
 AHD ChebOut<D> cheb_eval(const DevSlot& S, D t) {
     const int idx = record_index(S, t);
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -287,7 +290,7 @@ AHD ChebOut<D> cheb_eval(const DevSlot& S, D t) {
 #else
     const double* rec = S.rec + (size_t)idx * kRecWords;
 #endif
-    return cheb_series<D, Vel, Acc>(S, rec, t);
+    return cheb_series<D, Vel, Acc, NP>(S, rec, t);
 }
 
 // ---- substep ephemeris cache and wave-wide Chebyshev rounds (8-lane groups) ---------------
@@ -353,7 +356,7 @@ __device__ __forceinline__ void ec_prefetch(SubstepCache& sc, int lane, int n) {
 // lane 0, the Sun's raw velocity, for 8-lane groups (see the cache description above).  All
 // active lanes of the wave call this together; whole groups are active or not.  The evaluating
 // lanes write each result to the wave's LDS exchange area, from which the owners read it.
-__device__ __forceinline__ void wave_positions(const DevSlot* slots, int lane, double t, bool want_vel,
+__device__ __forceinline__ void wave_positions(const DevSlot* slots, int short_from, int lane, double t, bool want_vel,
                                                const SubstepCache& sc, double px[4][3], double sv[3]) {
     constexpr unsigned long long kLeaders = 0x0101010101010101ULL;
     const int wl = __lane_id();
@@ -381,9 +384,12 @@ __device__ __forceinline__ void wave_positions(const DevSlot* slots, int lane, d
         const int gi = nth_group(nd, i < k ? i : 0);
         const double ti = __shfl(t, 8 * gi, 64);
         const int nm = __shfl(sc.n * 4 + sc.mode, 8 * gi, 64);
+        // a round whose slots all have at most kCoefShort coefficients runs the short series
+        const bool short_round = __ballot(j < items && s < short_from) == 0ULL;
         if (j < items) {
             ChebOut<double> c;
             if (q == 0 && any_vel) c = cheb_eval<double, true>(slots[s], ti);
+            else if (short_round) c = cheb_eval<double, false, false, kCoefShort>(slots[s], ti);
             else c = cheb_eval<double, false>(slots[s], ti);
             double* e = x + (gi * kXchRows + s) * 3;
             e[0] = c.x; e[1] = c.y; e[2] = c.z;
@@ -971,7 +977,7 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
 #if defined(__HIP_DEVICE_COMPILE__)
     if constexpr (G == 8 && !Ext && std::is_same<D, double>::value) {
         if (sc) {
-            wave_positions(slots, lane, t, want_sun_vel, *sc, px, sunv_l);
+            wave_positions(slots, E.short_from, lane, t, want_sun_vel, *sc, px, sunv_l);
             from_wave = true;
         }
     }
@@ -1125,10 +1131,7 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
         // one system per lane: each slot is evaluated and folded in at once (the Sun and the
         // EMB, which the others need, were evaluated above); four slots in flight (3% faster
         // than two on 1,048,576 real-only systems)
-#pragma unroll 4
-        for (int s = 1; s < kNumSlots; s++) {
-            if (s == kSlotEMB) continue;
-            const ChebOut<D> c = cheb_eval<D, false>(slots[s], t);
+        auto fold = [&](int s, const ChebOut<D>& c) {
             D pk[3];
             if (s < kFirstAsteroidSlot) {
                 const bool em = (s == kSlotEarth || s == kSlotMoon);
@@ -1143,6 +1146,25 @@ AHD void force_eval(const DevEphem& E, const DevSlot* slots, const double* gm, c
             }
             const int b = slot_body(s);
             if (b < nbody) body_term(b, pk);
+        };
+#pragma unroll 4
+        for (int s = 1; s < kFirstAsteroidSlot; s++) {
+            if (s == kSlotEMB) continue;
+            fold(s, cheb_eval<D, false>(slots[s], t));
+        }
+        // the asteroids with the short series where every asteroid record allows it (the bound
+        // is the same on every lane: a scalar branch)
+#if defined(__HIP_DEVICE_COMPILE__)
+        const int short_from = __builtin_amdgcn_readfirstlane(E.short_from);
+#else
+        const int short_from = E.short_from;
+#endif
+        if (short_from <= kFirstAsteroidSlot) {
+#pragma unroll 4
+            for (int s = kFirstAsteroidSlot; s < kNumSlots; s++) fold(s, cheb_eval<D, false, false, kCoefShort>(slots[s], t));
+        } else {
+#pragma unroll 4
+            for (int s = kFirstAsteroidSlot; s < kNumSlots; s++) fold(s, cheb_eval<D, false>(slots[s], t));
         }
         body_term(0, sun);
     } else {
