@@ -419,10 +419,11 @@ def comment_lines(model, constants=None):
 
 
 def write_synthetic_ephemeris(planets_path, asteroids_path, seed=0, planet_mass_scale=1.0, constants=None,
-                              asteroid_coverage_days=COVERAGE_DAYS):
+                              asteroid_coverage_days=COVERAGE_DAYS, asteroid_record=ASTEROID_RECORD):
     """Write the synthetic planets and asteroid SPK files; returns the model.  ``constants``
     overrides entries of the comment-area constants (tests isolate J2/J3/J4 this way);
-    ``asteroid_coverage_days`` shortens the asteroid file's coverage (JD 2451545 +/- that)."""
+    ``asteroid_coverage_days`` shortens the asteroid file's coverage (JD 2451545 +/- that);
+    ``asteroid_record`` is the asteroid records' (length in days, coefficients per coordinate)."""
     model = SyntheticModel(seed, planet_mass_scale)
     planet_segs = [_segment(model, 10, 0, PLANET_RECORD)]
     for naif in (1, 2, 3, 4, 5, 6, 7, 8, 9):
@@ -431,7 +432,7 @@ def write_synthetic_ephemeris(planets_path, asteroids_path, seed=0, planet_mass_
     planet_segs.append(_segment(model, 301, 3, MOON_RECORD))
     lines = comment_lines(model, constants)
     write_spk(planets_path, planet_segs, lines, "ASSIST-AMD SYNTHETIC PLANETS")
-    ast_segs = [_segment(model, 2000000 + a["num"], 10, ASTEROID_RECORD, asteroid_coverage_days)
+    ast_segs = [_segment(model, 2000000 + a["num"], 10, asteroid_record, asteroid_coverage_days)
                 for a in model.asteroids]
     write_spk(asteroids_path, ast_segs, lines[:2], "ASSIST-AMD SYNTHETIC ASTEROIDS")
     return model

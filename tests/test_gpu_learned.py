@@ -6,8 +6,9 @@
 * Adam training on the GPU fits a teacher network;
 * the feature kernel against features computed from the oracle's ephemeris;
 * step traces of the plain integrator are consistent with its step counts;
-* the trained network (assist_amd/models/step_mlp.npz) against the plain start: IAS15 still
-  judges every step, so final states agree with the plain run to the integrator's accuracy,
+* the trained network (assist_amd/models/step_mlp.npz) against the plain start: IAS15's own
+  error test still accepts or rejects every step, so final states agree with the plain run to
+  the integrator's accuracy,
   and the learned start saves force evaluations.
 """
 import os
