@@ -241,8 +241,14 @@ int get_device_ephem(const assist_ephem* e, const DevEphem** out, DevEphem* host
         H.ta_end = n_ast > 0 ? ae - e->jd_ref : H.t_end;
     }
     H.n_ast = n_ast;
+    // first slot from which every slot can run the short series: its records have at most
+    // kCoefShort coefficients, or it is a missing asteroid's (whose positions no term uses)
     H.short_from = kNumSlots;
-    for (int s = kNumSlots - 1; s >= 0 && H.slot[s].P <= kCoefShort; s--) H.short_from = s;
+    for (int s = kNumSlots - 1; s >= 0; s--) {
+        const bool unused = s >= kFirstAsteroidSlot + n_ast;
+        if (!unused && H.slot[s].P > kCoefShort) break;
+        H.short_from = s;
+    }
     H.J2E = e->J2E; H.J3E = e->J3E; H.J4E = e->J4E; H.J2SUN = e->J2SUN;
     H.Re_eq = e->Re_eq; H.Rs_eq = e->Rs_eq; H.c_squared = e->c_squared; H.over_c_squared = e->over_c_squared;
     const double d2r = M_PI / 180.;
